@@ -1,0 +1,147 @@
+/*
+ * ugemm_hip.h -- C ABI of libugemm_hip.so, an MI355X (gfx950) GEMM backend
+ * for the ugemm C API.
+ *
+ * Semantics (all entry points):
+ *   C <- alpha * op(A) * op(B) + beta * C
+ *   major 'R' (row-major) or 'C' (column-major); transa/transb 'N' or 'T'
+ *   (any value other than 'N'/'n' means transpose, as in gemm_cpu.h:327-360).
+ *   Row-major 'N' A is M x K (lda >= K); 'T' A is stored K x M (lda >= M).
+ *   Row-major 'N' B is K x N (ldb >= N); 'T' B is stored N x K (ldb >= K).
+ *   C is M x N (ldc >= N).  Column-major swaps the roles of rows and columns
+ *   exactly as ugemm.h:259-285 (dgemm_cpu) indexes them.
+ *   beta == 0 never reads C (BLAS rule; gemm_cpu.h:119-123, sgemm_ocl2.h:86-88).
+ *   alpha == 0 or K == 0 scales C by beta with the correct major indexing.
+ *
+ * Errors: functions returning int return UGEMM_OK (0) or a negative
+ * ugemm_status_t; the void uut-style entry points record the status, which
+ * ugemm_hip_last_error() returns (per calling thread).
+ *
+ * Streams are passed as `void *` (a hipStream_t, or NULL for the null stream)
+ * so that this header needs no HIP include.
+ */
+#ifndef UGEMM_HIP_H
+#define UGEMM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    UGEMM_OK = 0,
+    UGEMM_ERR_ARG = -1,         /* invalid argument (bad ld, NULL pointer, bad flag) */
+    UGEMM_ERR_HIP = -2,         /* a HIP runtime call failed */
+    UGEMM_ERR_NOMEM = -3,       /* device or host allocation failed */
+    UGEMM_ERR_NODEV = -4,       /* no usable gfx950 device */
+    UGEMM_ERR_UNSUPPORTED = -5, /* mode/layout combination not provided */
+    UGEMM_ERR_TUNE_FILE = -6    /* tuning cache could not be read or written */
+} ugemm_status_t;
+
+/* Arithmetic forms of the SGEMM path. */
+typedef enum {
+    UGEMM_MODE_FP32 = 0,      /* plain FP32: vector FMA, FP32 accumulate */
+    UGEMM_MODE_BF16X3 = 1,    /* 2-way BF16 split, hi*hi + hi*lo + lo*hi on MFMA, FP32 accumulate */
+    UGEMM_MODE_BF16X1 = 2,    /* BF16 round-to-nearest operands, one MFMA product (opt-in, low precision) */
+    UGEMM_MODE_FP8_BLOCK = 3, /* FP8-e4m3 operands with one FP32 scale per 128x128 block (opt-in) */
+    UGEMM_MODE_BF16X6 = 4,    /* 3-way BF16 split, six MFMA products: FP32-level error at large K */
+    UGEMM_MODE_COUNT = 5
+} ugemm_mode_t;
+
+/* ------------------------------------------------------------------ */
+/* 1. uut-compatible entry points: host pointers, synchronous.         */
+/*    Replace the function pointer handed to test_sgemm/test_dgemm     */
+/*    (check_sgemm.c:87-103, check_dgemm.c:86-102); same argument list  */
+/*    as sgemm_cpu/dgemm_cpu (ugemm.h:162-176, ugemm.h:287-301).       */
+/*    sgemm_hip runs UGEMM_MODE_FP32 unless UGEMM_HIP_MODE is set      */
+/*    (fp32|bf16x3|bf16x1|fp8|bf16x6).                                 */
+/* ------------------------------------------------------------------ */
+void sgemm_hip(char major, char transa, char transb, int M, int N, int K,
+               float alpha, const float *A, int lda, const float *B, int ldb,
+               float beta, float *C, int ldc);
+void dgemm_hip(char major, char transa, char transb, int M, int N, int K,
+               double alpha, const double *A, int lda, const double *B, int ldb,
+               double beta, double *C, int ldc);
+
+/* Host-pointer SGEMM with an explicit arithmetic mode. */
+int ugemm_hip_sgemm(char major, char transa, char transb, int M, int N, int K,
+                    float alpha, const float *A, int lda, const float *B, int ldb,
+                    float beta, float *C, int ldc, int mode);
+
+/* ------------------------------------------------------------------ */
+/* 2. GPU-backend trio, as sgemm_ocl_init / sgemm_ocl / sgemm_ocl_finish */
+/*    (sgemm_ocl2.h:158-224, selected by sgemm_test.c:29-33).          */
+/* ------------------------------------------------------------------ */
+/* Select the device (-1: $UGEMM_HIP_DEVICE or 0) and reserve max_bytes of
+ * device staging memory for the host-pointer entry points (0: grow on demand).
+ * Calling any entry point without init initialises device 0 lazily. */
+int ugemm_hip_init(int device, size_t max_bytes);
+void ugemm_hip_finish(void);
+/* Packed row-major SGEMM (lda = K or M, ldb = N or K, ldc = N by transposition),
+ * the argument list of sgemm_ocl (sgemm_ocl2.h:166). */
+void sgemm_hip_packed(char ta, char tb, int m, int n, int k, float alpha,
+                      float *a, float *b, float beta, float *c);
+
+/* ------------------------------------------------------------------ */
+/* 3. Device-pointer, stream-ordered entry points (no host sync).      */
+/* ------------------------------------------------------------------ */
+int ugemm_hip_sgemm_dev(char major, char transa, char transb, int M, int N, int K,
+                        float alpha, const float *dA, int lda, const float *dB, int ldb,
+                        float beta, float *dC, int ldc, int mode, void *stream);
+int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int K,
+                        double alpha, const double *dA, int lda, const double *dB, int ldb,
+                        double beta, double *dC, int ldc, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 4. Grouped (batched, uneven) SGEMM: one persistent launch.          */
+/*    Every problem is row-major; pointers are device pointers.        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    char transa, transb;
+    int M, N, K;
+    float alpha, beta;
+    const float *A; int lda;
+    const float *B; int ldb;
+    float *C; int ldc;
+} ugemm_problem_t;
+
+typedef struct ugemm_grouped_plan ugemm_grouped_plan_t;
+
+/* Build a plan: validates the problems, builds the tile table and the static
+ * per-workgroup schedule, uploads them, and (capture != 0) records the launch
+ * in a hipGraph so that ugemm_hip_grouped_run replays one graph launch. */
+int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
+                                  const ugemm_problem_t *problems, int mode,
+                                  int capture, void *stream);
+int ugemm_hip_grouped_run(ugemm_grouped_plan_t *plan, void *stream);
+void ugemm_hip_grouped_plan_destroy(ugemm_grouped_plan_t *plan);
+/* One-shot convenience: plan, run (no capture), destroy. */
+int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode,
+                            void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 5. Status and tuning.                                              */
+/* ------------------------------------------------------------------ */
+int ugemm_hip_last_error(void);
+const char *ugemm_hip_last_error_message(void);
+const char *ugemm_hip_status_string(int status);
+/* Time every compiled kernel variant on an (M, N, K, layout, mode) problem
+ * and remember the fastest for that shape bucket. Allocates its own buffers. */
+int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode);
+int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *plan);
+int ugemm_hip_tune_load(const char *path);
+int ugemm_hip_tune_save(const char *path);
+/* Variant chosen for a problem (index into the mode's kernel table); -1 = none. */
+int ugemm_hip_tune_lookup(char transa, char transb, int M, int N, int K, int mode);
+/* Force a variant for the next calls on this thread (-1 restores tuning/heuristics). */
+int ugemm_hip_force_variant(int mode, int variant);
+int ugemm_hip_variant_count(int mode);
+const char *ugemm_hip_variant_name(int mode, int variant);
+const char *ugemm_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UGEMM_HIP_H */

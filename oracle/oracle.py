@@ -1,0 +1,219 @@
+"""CPU oracle for the GEMM path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module; the product library never does.
+
+Contents
+  * ctypes bindings to liboracle.so (oracle.c: the naive loops of
+    ugemm.h:162-411, FP64 / double-double references with the error scale,
+    the gemm_cpu.h blocking) and, when built, to oracle/_ref/libugemm_ref.so
+    (the reference's own ugemm.h compiled by oracle/Makefile);
+  * input generators for the workload distributions (numpy PCG64, seeded);
+  * cmp_results: the error statistic of check_sgemm.c:56-85;
+  * acceptance checks: |C - C_ref| <= tol * scale elementwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libugemm_ref.so")
+
+EPS32 = 2.0 ** -24   # unit roundoff of binary32
+EPS64 = 2.0 ** -53
+
+_o = None
+_r = None
+
+
+def _c(x):
+    return x.encode() if isinstance(x, str) else x
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            raise ImportError(f"{ORACLE_SO} missing: run make -C oracle")
+        L = ctypes.CDLL(ORACLE_SO)
+        c, i, f, d, vp = ctypes.c_char, ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+        for name, T in (("oracle_sgemm_naive", f), ("oracle_sgemm_packed", f)):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [c, c, c, i, i, i, T, vp, i, vp, i, T, vp, i]
+        for name in ("oracle_dgemm_naive", "oracle_dgemm_packed"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i]
+        for name in ("oracle_sgemm_f64", "oracle_dgemm_dd"):
+            fn = getattr(L, name)
+            fn.restype = i
+            fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp, i, vp, vp, i]
+        _o = L
+    return _o
+
+
+def ref_lib():
+    """The reference's CPU library compiled from its sources (None if not built)."""
+    global _r
+    if _r is None:
+        if not os.path.exists(REF_SO):
+            return None
+        L = ctypes.CDLL(REF_SO)
+        c, i, f, d, vp = ctypes.c_char, ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+        for name in ("sgemm_cpu", "sgemm_c", "sgemm_avx", "ref_sgemm_sse"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i]
+        for name in ("dgemm_cpu", "_dgemm_c", "dgemm_avx"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i]
+        _r = L
+    return _r
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# Calls into oracle.c
+
+def sgemm_naive(major, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    """sgemm_cpu semantics (ugemm.h:287-411); C is updated in place."""
+    assert A.dtype == B.dtype == C.dtype == np.float32
+    lib().oracle_sgemm_naive(_c(major), _c(ta), _c(tb), M, N, K, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+
+
+def dgemm_naive(major, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    assert A.dtype == B.dtype == C.dtype == np.float64
+    lib().oracle_dgemm_naive(_c(major), _c(ta), _c(tb), M, N, K, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+
+
+def sgemm_packed(major, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    lib().oracle_sgemm_packed(_c(major), _c(ta), _c(tb), M, N, K, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+
+
+def dgemm_packed(major, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    lib().oracle_dgemm_packed(_c(major), _c(ta), _c(tb), M, N, K, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+
+
+def reference_f64(major, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C0, ldc, rows=None, nthreads=0):
+    """Accurate result and error scale.
+
+    float32 operands: compensated FP64 sums; float64 operands: double-double.
+    Returns (out, scale) as float64 arrays of shape (len(rows) or M, N) for
+    row-major, or (M, N) indexed [m, n] for column-major too (C[m, n]).
+    `rows` selects rows m of C (row-major) -- used to sample large shapes.
+    """
+    dbl = A.dtype == np.float64
+    fn = lib().oracle_dgemm_dd if dbl else lib().oracle_sgemm_f64
+    r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nr = M if r is None else len(r)
+    out = np.empty((nr, N), np.float64)
+    scale = np.empty((nr, N), np.float64)
+    c0 = C0 if (C0 is not None and beta != 0) else None
+    rc = fn(_c(major), _c(ta), _c(tb), M, N, K, float(alpha), _p(A), lda, _p(B), ldb, float(beta), _p(c0), ldc,
+            _p(r), nr, _p(out), _p(scale), nthreads)
+    if rc != 0:
+        raise ValueError("oracle: bad flags")
+    return out, scale
+
+
+# ---------------------------------------------------------------------------
+# Inputs
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def uniform(g, shape, lo=-1.0, hi=1.0, dtype=np.float32):
+    """U[lo, hi) built from 24 random bits: exact in binary32 for [-1, 1)."""
+    k = g.integers(0, 1 << 24, size=shape, dtype=np.int64).astype(np.float64)
+    return (lo + (hi - lo) * k * 2.0 ** -24).astype(dtype)
+
+
+def normal(g, shape, dtype=np.float32):
+    return g.standard_normal(size=shape).astype(dtype)
+
+
+def wide_range(g, shape, emin=-12, emax=12, dtype=np.float32):
+    """sign * 2^e * (1 + u), e ~ U{emin..emax}, u ~ U[0,1) on 23 bits (exact)."""
+    s = np.where(g.integers(0, 2, size=shape) == 0, -1.0, 1.0)
+    e = g.integers(emin, emax + 1, size=shape)
+    u = g.integers(0, 1 << 23, size=shape).astype(np.float64) * 2.0 ** -23
+    return (s * np.ldexp(1.0 + u, e)).astype(dtype)
+
+
+def stored_shape(major, trans, rows, cols, ld):
+    """Shape of the buffer holding op(X) (rows x cols) with leading dimension ld.
+
+    Row-major 'N' stores rows x ld; 'T' stores cols x ld.  Column-major is
+    the transpose of that picture (buffer of  cols x ld for 'N')."""
+    t = trans != "N"
+    if major == "R":
+        return (cols, ld) if t else (rows, ld)
+    return (rows, ld) if t else (cols, ld)
+
+
+def op_matrix(major, trans, X, rows, cols):
+    """The logical rows x cols matrix op(X) seen through a stored buffer X."""
+    t = trans != "N"
+    if major == "R":
+        return X[:cols, :rows].T if t else X[:rows, :cols]
+    return X[:rows, :cols] if t else X[:cols, :rows].T
+
+
+def c_matrix(major, C, M, N):
+    return C[:M, :N] if major == "R" else C[:N, :M].T
+
+
+# ---------------------------------------------------------------------------
+# Statistics and acceptance
+
+def cmp_results(M, N, ref, res, ld):
+    """check_sgemm.c:56-85: (stdErr, stdRef, maxErr, argmax (m, n), verdict)."""
+    r = ref.reshape(-1, ld)[:M, :N].astype(np.float64)
+    x = res.reshape(-1, ld)[:M, :N].astype(np.float64)
+    err = x - r
+    a = np.abs(err)
+    i = int(np.argmax(a))
+    max_err = float(a.flat[i])
+    std_err = float(np.sqrt((err * err).sum() / (M * N)))
+    s1, s2 = float(r.sum()), float((r * r).sum())
+    std_ref = float(np.sqrt(max(s2 * (M * N) - s1 * s1, 0.0)) / (M * N))
+    verdict = "FAIL !!!" if max_err > std_ref * 1e-5 else (
+        "Sucks !" if (max_err > std_ref * 3e-5 or std_err > std_ref * 1e-6) else "")
+    return std_err, std_ref, max_err, divmod(i, N), verdict
+
+
+def tolerance(K, eps=EPS32, extra=0.0):
+    """Elementwise acceptance factor: |C - C_exact| <= tolerance * scale.
+
+    2*K*eps covers sequential accumulation of K products (the bound used for
+    the plain path); 2*eps more covers the alpha/beta epilogue roundings;
+    `extra` adds a per-product term for reduced-precision operand forms."""
+    return 2.0 * (K + 1) * eps + extra
+
+
+def check_bound(res, exact, scale, tol):
+    """Returns (ok, worst ratio |res-exact| / (tol*scale))."""
+    res = np.asarray(res, np.float64)
+    err = np.abs(res - exact)
+    lim = tol * scale
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = np.where(lim > 0, err / lim, np.where(err > 0, np.inf, 0.0))
+    worst = float(np.nanmax(ratio)) if ratio.size else 0.0
+    ok = bool(np.all(np.isfinite(res) == np.isfinite(exact))) and worst <= 1.0
+    return ok, worst
+
+
+def rel_errors(res, exact):
+    """max and mean of |res - exact| / |exact| over entries with exact != 0."""
+    res = np.asarray(res, np.float64)
+    nz = exact != 0
+    rel = np.abs(res[nz] - exact[nz]) / np.abs(exact[nz])
+    return (float(rel.max()), float(rel.mean())) if rel.size else (0.0, 0.0)

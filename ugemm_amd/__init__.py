@@ -1,0 +1,194 @@
+"""ugemm_amd: Python access to libugemm_hip.so, the MI355X (gfx950) GEMM
+backend for the ugemm C API.
+
+The library is the product; this module only loads it with ctypes and gives
+Python callers the same argument lists as the C ABI (include/ugemm_hip.h):
+
+    sgemm(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)
+
+mirrors the uut signature of sgemm_cpu / sgemm_avx (ugemm.h:287-301) on host
+numpy arrays, and sgemm_dev(...) the device-pointer, stream-ordered entry.
+Errors raise UgemmError carrying the library's status code and message.
+There is no CPU fallback: if libugemm_hip.so is missing, importing the
+bindings (load()) raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libugemm_hip.so")
+
+MODE_FP32, MODE_BF16X3, MODE_BF16X1, MODE_FP8_BLOCK, MODE_BF16X6 = 0, 1, 2, 3, 4
+MODE_DGEMM = -1  # variant tables of the DGEMM family
+MODES = {"fp32": MODE_FP32, "bf16x3": MODE_BF16X3, "bf16x1": MODE_BF16X1,
+         "fp8": MODE_FP8_BLOCK, "bf16x6": MODE_BF16X6}
+
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_UNSUPPORTED, ERR_TUNE_FILE = 0, -1, -2, -3, -4, -5, -6
+
+_lib = None
+
+
+class UgemmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"ugemm_hip status {status}: {message}")
+        self.status = status
+
+
+class Problem(ctypes.Structure):
+    """ugemm_problem_t (include/ugemm_hip.h)."""
+    _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char),
+                ("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int)]
+
+
+def load(path=None):
+    """Loads libugemm_hip.so (once) and declares every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(path)
+    c, i, f, d, vp, sz = ctypes.c_char, ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+    cs = ctypes.c_char_p
+    sig = {
+        "sgemm_hip": (None, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i]),
+        "dgemm_hip": (None, [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i]),
+        "ugemm_hip_sgemm": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i]),
+        "ugemm_hip_init": (i, [i, sz]),
+        "ugemm_hip_finish": (None, []),
+        "sgemm_hip_packed": (None, [c, c, i, i, i, f, vp, vp, f, vp]),
+        "ugemm_hip_sgemm_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp]),
+        "ugemm_hip_dgemm_dev": (i, [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
+        "ugemm_hip_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(Problem), i, i, vp]),
+        "ugemm_hip_grouped_run": (i, [vp, vp]),
+        "ugemm_hip_grouped_plan_destroy": (None, [vp]),
+        "ugemm_hip_sgemm_grouped": (i, [i, ctypes.POINTER(Problem), i, vp]),
+        "ugemm_hip_last_error": (i, []),
+        "ugemm_hip_last_error_message": (cs, []),
+        "ugemm_hip_status_string": (cs, [i]),
+        "ugemm_hip_tune_sgemm": (i, [c, c, i, i, i, i]),
+        "ugemm_hip_tune_grouped": (i, [vp]),
+        "ugemm_hip_tune_load": (i, [cs]),
+        "ugemm_hip_tune_save": (i, [cs]),
+        "ugemm_hip_tune_lookup": (i, [c, c, i, i, i, i]),
+        "ugemm_hip_force_variant": (i, [i, i]),
+        "ugemm_hip_variant_count": (i, [i]),
+        "ugemm_hip_variant_name": (cs, [i, i]),
+        "ugemm_hip_version": (cs, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _b(ch):
+    return ch.encode() if isinstance(ch, str) else ch
+
+
+def _raise_if(rc):
+    if rc != OK:
+        L = load()
+        raise UgemmError(rc, L.ugemm_hip_last_error_message().decode())
+    return rc
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    if hasattr(x, "ptr"):
+        return x.ptr
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def init(device=-1, max_bytes=0):
+    return _raise_if(load().ugemm_hip_init(device, max_bytes))
+
+
+def finish():
+    load().ugemm_hip_finish()
+
+
+def sgemm(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode=MODE_FP32):
+    """Host-pointer SGEMM (synchronous); C (float32 numpy array) is updated in place."""
+    L = load()
+    rc = L.ugemm_hip_sgemm(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb,
+                           beta, _ptr(C), ldc, mode)
+    return _raise_if(rc)
+
+
+def dgemm(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc):
+    """Host-pointer DGEMM through the void uut entry dgemm_hip; raises on error."""
+    L = load()
+    L.dgemm_hip(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C), ldc)
+    return _raise_if(L.ugemm_hip_last_error())
+
+
+def sgemm_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, mode=MODE_FP32, stream=None):
+    """Device-pointer SGEMM on `stream` (a ugemm_amd.hip.Stream, a raw handle, or None)."""
+    s = stream.handle if hasattr(stream, "handle") else stream
+    rc = load().ugemm_hip_sgemm_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
+                                    _ptr(dB), ldb, beta, _ptr(dC), ldc, mode, s)
+    return _raise_if(rc)
+
+
+def dgemm_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, stream=None):
+    s = stream.handle if hasattr(stream, "handle") else stream
+    rc = load().ugemm_hip_dgemm_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
+                                    _ptr(dB), ldb, beta, _ptr(dC), ldc, s)
+    return _raise_if(rc)
+
+
+def variants(mode):
+    L = load()
+    return [L.ugemm_hip_variant_name(mode, v).decode() for v in range(L.ugemm_hip_variant_count(mode))]
+
+
+def force_variant(mode, variant):
+    return _raise_if(load().ugemm_hip_force_variant(mode, variant))
+
+
+class GroupedPlan:
+    """A planned grouped SGEMM (one persistent launch, optionally a hipGraph)."""
+
+    def __init__(self, problems, mode=MODE_FP32, capture=True, stream=None):
+        self._arr = (Problem * len(problems))(*problems)
+        h = ctypes.c_void_p()
+        s = stream.handle if hasattr(stream, "handle") else stream
+        _raise_if(load().ugemm_hip_grouped_plan_create(ctypes.byref(h), len(problems), self._arr, mode,
+                                                       1 if capture else 0, s))
+        self.handle = h.value
+
+    def run(self, stream=None):
+        s = stream.handle if hasattr(stream, "handle") else stream
+        return _raise_if(load().ugemm_hip_grouped_run(self.handle, s))
+
+    def tune(self):
+        rc = load().ugemm_hip_tune_grouped(self.handle)
+        if rc < 0:
+            _raise_if(rc)
+        return rc
+
+    def destroy(self):
+        if self.handle:
+            load().ugemm_hip_grouped_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,75 @@
+// Internal declarations shared by the libugemm_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ugemm_hip.h"
+
+namespace ugemm {
+
+// One row-major GEMM after major/transpose normalisation:
+//   C[m, n] = alpha * sum_k opA(m, k) * opB(k, n) + beta * C[m, n]
+//   opA(m, k) = ta ? A[k * lda + m] : A[m * lda + k]
+//   opB(k, n) = tb ? B[n * ldb + k] : B[k * ldb + n]
+template <typename T>
+struct Gemm {
+    int M, N, K;
+    T alpha, beta;
+    const T *A; long lda;
+    const T *B; long ldb;
+    T *C; long ldc;
+    bool ta, tb;
+};
+
+// ---- status (runtime.hip) ----
+int fail(int status, const char *fmt, ...);     // records status + message, returns status
+int hip_fail(hipError_t e, const char *what);   // records UGEMM_ERR_HIP
+#define UGEMM_HIP_TRY(expr)                                   \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return ::ugemm::hip_fail(_e, #expr); \
+    } while (0)
+
+int ensure_device();        // lazily initialises the device; UGEMM_OK or error
+int device_cu_count();
+
+// ---- kernel families (one launcher per family) ----
+// Variant tables: each family exposes a count, names, and a launcher.
+int fp32_variant_count();
+const char *fp32_variant_name(int v);
+int fp32_pick_variant(const Gemm<float> &g);
+int launch_fp32(const Gemm<float> &g, int variant, hipStream_t s);
+
+int mfma_variant_count(int mode);
+const char *mfma_variant_name(int mode, int v);
+int mfma_pick_variant(const Gemm<float> &g, int mode);
+int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
+
+int fp8_variant_count();
+const char *fp8_variant_name(int v);
+int launch_fp8(const Gemm<float> &g, int variant, hipStream_t s);
+
+int dgemm_variant_count();
+const char *dgemm_variant_name(int v);
+int dgemm_pick_variant(const Gemm<double> &g);
+int launch_dgemm(const Gemm<double> &g, int variant, hipStream_t s);
+
+// C <- beta * C for the alpha == 0 / K == 0 cases (beta == 0 writes zeros).
+int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s);
+int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s);
+
+// ---- tuning (tune.cpp) ----
+int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K);   // -1 if none
+int forced_variant(int mode);                                         // -1 if none
+void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms);
+
+// ---- device scratch (runtime.hip) ----
+// Returns a device buffer of at least `bytes` bytes, owned by the runtime and
+// reused across calls on the calling stream's device. Not for host-pointer
+// staging (see staging_buffer).
+int workspace(size_t bytes, void **ptr);
+
+}  // namespace ugemm
+
+// Alignment helpers used by launchers to select vectorised variants.
+static inline bool ugemm_aligned16(const void *p) { return (((uintptr_t)p) & 15u) == 0; }

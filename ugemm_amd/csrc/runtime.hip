@@ -1,0 +1,413 @@
+// Runtime of libugemm_hip.so: status reporting, device set-up, argument
+// normalisation, host-pointer staging, and the exported entry points.
+//
+// Boundary replaced (reference -> here):
+//   sgemm_cpu / sgemm_c / sgemm_avx uut signature (ugemm.h:287-301,
+//   check_sgemm.c:96-103)                  -> sgemm_hip, ugemm_hip_sgemm
+//   dgemm_cpu / dgemm_avx (ugemm.h:162-176) -> dgemm_hip
+//   sgemm_ocl_init / sgemm_ocl / sgemm_ocl_finish (sgemm_ocl2.h:158-224)
+//                                          -> ugemm_hip_init / sgemm_hip_packed / ugemm_hip_finish
+#include "internal.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+
+namespace ugemm {
+
+// ---------------------------------------------------------------------------
+// Status: per-thread code + message.
+static thread_local int t_status = UGEMM_OK;
+static thread_local char t_msg[256] = "";
+
+int fail(int status, const char *fmt, ...) {
+    t_status = status;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof t_msg, fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? UGEMM_ERR_NOMEM : UGEMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+static int ok() {
+    t_status = UGEMM_OK;
+    t_msg[0] = 0;
+    return UGEMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device state. One device per process image of the library; the host-pointer
+// entry points run on it, the device-pointer ones on the caller's stream.
+struct DeviceState {
+    std::mutex mu;              // guards the fields below and the staging buffer
+    bool ready = false;
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;   // internal stream for host-pointer calls
+    void *staging = nullptr;        // device staging for host-pointer calls
+    size_t staging_bytes = 0;
+};
+static DeviceState g_dev;
+
+static int init_locked(int device, size_t max_bytes) {
+    if (g_dev.ready) {
+        if (device >= 0 && device != g_dev.device)
+            return fail(UGEMM_ERR_ARG, "ugemm_hip_init: already initialised on device %d", g_dev.device);
+    } else {
+        if (device < 0) {
+            const char *e = getenv("UGEMM_HIP_DEVICE");
+            device = e ? atoi(e) : 0;
+        }
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(UGEMM_ERR_NODEV, "no HIP device");
+        if (device >= n) return fail(UGEMM_ERR_NODEV, "device %d of %d", device, n);
+        UGEMM_HIP_TRY(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        UGEMM_HIP_TRY(hipGetDeviceProperties(&prop, device));
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(UGEMM_ERR_NODEV, "device %d is %s; this library is built for gfx950 only", device,
+                        prop.gcnArchName);
+        g_dev.device = device;
+        g_dev.cus = prop.multiProcessorCount;
+        UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        g_dev.ready = true;
+    }
+    if (max_bytes > g_dev.staging_bytes) {
+        if (g_dev.staging) {
+            UGEMM_HIP_TRY(hipFree(g_dev.staging));
+            g_dev.staging = nullptr;
+            g_dev.staging_bytes = 0;
+        }
+        UGEMM_HIP_TRY(hipMalloc(&g_dev.staging, max_bytes));
+        g_dev.staging_bytes = max_bytes;
+    }
+    return UGEMM_OK;
+}
+
+int ensure_device() {
+    if (g_dev.ready) return UGEMM_OK;
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    return init_locked(-1, 0);
+}
+
+int device_cu_count() { return g_dev.ready ? g_dev.cus : 256; }
+
+int workspace(size_t bytes, void **ptr) {
+    // Stream-ordered allocations are made by the callers with hipMallocAsync;
+    // this synchronous fallback is for set-up paths only.
+    UGEMM_HIP_TRY(hipMalloc(ptr, bytes));
+    return UGEMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C <- beta * C kernels (alpha == 0 or K == 0). beta == 0 writes zeros.
+template <typename T>
+__global__ void scale_kernel(int M, int N, T beta, T *__restrict__ C, long ldc) {
+    const long total = (long)M * N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long m = i / N, n = i - m * N;
+        T *p = C + m * ldc + n;
+        *p = (beta == T(0)) ? T(0) : beta * *p;
+    }
+}
+
+template <typename T>
+static int launch_scale(int M, int N, T beta, T *C, long ldc, hipStream_t s) {
+    const long total = (long)M * N;
+    const int threads = 256;
+    long blocks = (total + threads - 1) / threads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0, s, M, N, beta, C, ldc);
+    UGEMM_HIP_TRY(hipGetLastError());
+    return UGEMM_OK;
+}
+int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s) { return launch_scale(M, N, beta, C, ldc, s); }
+int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s) { return launch_scale(M, N, beta, C, ldc, s); }
+
+// ---------------------------------------------------------------------------
+// Argument normalisation: validate, then express every call as a row-major
+// GEMM. Column-major C (M x N) is row-major C^T (N x M) = op(B)^T op(A)^T, so
+// a column-major call maps to row-major (transb, transa, N, M, K, B, A).
+static int parse_trans(char t, bool *out) {
+    switch (t) {
+        case 'N': case 'n': *out = false; return 1;
+        case 'T': case 't': case 'C': case 'c': *out = true; return 1;
+        default: return 0;
+    }
+}
+
+template <typename T>
+static int normalize(char major, char transa, char transb, int M, int N, int K, T alpha, const T *A, int lda,
+                     const T *B, int ldb, T beta, T *C, int ldc, Gemm<T> &g) {
+    bool ta, tb, row;
+    if (major == 'R' || major == 'r') row = true;
+    else if (major == 'C' || major == 'c') row = false;
+    else return fail(UGEMM_ERR_ARG, "major must be 'R' or 'C' (got 0x%02x)", (unsigned char)major);
+    if (!parse_trans(transa, &ta)) return fail(UGEMM_ERR_ARG, "bad transa 0x%02x", (unsigned char)transa);
+    if (!parse_trans(transb, &tb)) return fail(UGEMM_ERR_ARG, "bad transb 0x%02x", (unsigned char)transb);
+    if (M < 0 || N < 0 || K < 0) return fail(UGEMM_ERR_ARG, "negative size M=%d N=%d K=%d", M, N, K);
+    // Leading dimensions as in BLAS: at least the length of a stored row
+    // (row-major) or column (column-major), and at least 1.
+    auto need = [](int x) { return x > 1 ? x : 1; };
+    const int la = row ? (ta ? M : K) : (ta ? K : M);
+    const int lb = row ? (tb ? K : N) : (tb ? N : K);
+    const int lc = row ? N : M;
+    if (lda < need(la)) return fail(UGEMM_ERR_ARG, "lda=%d < %d", lda, need(la));
+    if (ldb < need(lb)) return fail(UGEMM_ERR_ARG, "ldb=%d < %d", ldb, need(lb));
+    if (ldc < need(lc)) return fail(UGEMM_ERR_ARG, "ldc=%d < %d", ldc, need(lc));
+    if (M > 0 && N > 0) {
+        if (!C) return fail(UGEMM_ERR_ARG, "C is NULL");
+        if (K > 0 && alpha != T(0) && (!A || !B)) return fail(UGEMM_ERR_ARG, "A or B is NULL");
+    }
+    if (row) {
+        g = Gemm<T>{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ta, tb};
+    } else {
+        g = Gemm<T>{N, M, K, alpha, beta, B, ldb, A, lda, C, ldc, tb, ta};
+    }
+    return UGEMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device-pointer execution of a normalised GEMM.
+static int choose_variant(const Gemm<float> &g, int mode) {
+    int v = forced_variant(mode);
+    if (v >= 0) return v;
+    v = tuned_variant(mode, g.ta, g.tb, g.M, g.N, g.K);
+    if (v >= 0) return v;
+    switch (mode) {
+        case UGEMM_MODE_FP32: return fp32_pick_variant(g);
+        case UGEMM_MODE_FP8_BLOCK: return 0;
+        default: return mfma_pick_variant(g, mode);
+    }
+}
+
+int run_sgemm(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
+    if (g.M == 0 || g.N == 0) return UGEMM_OK;
+    if (g.alpha == 0.0f || g.K == 0) {
+        if (g.beta == 1.0f) return UGEMM_OK;
+        return launch_scale_f32(g.M, g.N, g.beta, g.C, g.ldc, s);
+    }
+    if (variant < 0) variant = choose_variant(g, mode);
+    switch (mode) {
+        case UGEMM_MODE_FP32: return launch_fp32(g, variant, s);
+        case UGEMM_MODE_BF16X1:
+        case UGEMM_MODE_BF16X3:
+        case UGEMM_MODE_BF16X6: return launch_mfma(g, mode, variant, s);
+        case UGEMM_MODE_FP8_BLOCK: return launch_fp8(g, variant, s);
+        default: return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    }
+}
+
+int run_dgemm(const Gemm<double> &g, int variant, hipStream_t s) {
+    if (g.M == 0 || g.N == 0) return UGEMM_OK;
+    if (g.alpha == 0.0 || g.K == 0) {
+        if (g.beta == 1.0) return UGEMM_OK;
+        return launch_scale_f64(g.M, g.N, g.beta, g.C, g.ldc, s);
+    }
+    if (variant < 0) {
+        variant = forced_variant(-1);
+        if (variant < 0) variant = dgemm_pick_variant(g);
+    }
+    return launch_dgemm(g, variant, s);
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer execution: stage A, B (and C when it is read) into the device
+// staging buffer with 2-D copies that keep the caller's leading dimensions,
+// run on the internal stream, copy C back row by row (the gaps between rows
+// of a padded C are never written), and wait.
+static size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <typename T>
+static int run_host(const Gemm<T> &h, int mode) {
+    int st = ensure_device();
+    if (st) return st;
+    if (h.M == 0 || h.N == 0) return UGEMM_OK;
+    if (h.beta == T(1) && (h.alpha == T(0) || h.K == 0)) return UGEMM_OK;
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    UGEMM_HIP_TRY(hipSetDevice(g_dev.device));
+    const bool need_ab = h.alpha != T(0) && h.K > 0;
+    const bool need_c = h.beta != T(0);
+    // Stored extents (rows x contiguous length) of the three operands.
+    const long a_rows = h.ta ? h.K : h.M, a_len = h.ta ? h.M : h.K;
+    const long b_rows = h.tb ? h.N : h.K, b_len = h.tb ? h.K : h.N;
+    const size_t a_bytes = need_ab ? round_up((size_t)a_rows * h.lda * sizeof(T), 256) : 0;
+    const size_t b_bytes = need_ab ? round_up((size_t)b_rows * h.ldb * sizeof(T), 256) : 0;
+    const size_t c_bytes = round_up((size_t)h.M * h.ldc * sizeof(T), 256);
+    const size_t total = a_bytes + b_bytes + c_bytes;
+    if (total > g_dev.staging_bytes) {
+        if (g_dev.staging) {
+            UGEMM_HIP_TRY(hipFree(g_dev.staging));
+            g_dev.staging = nullptr;
+            g_dev.staging_bytes = 0;
+        }
+        UGEMM_HIP_TRY(hipMalloc(&g_dev.staging, total));
+        g_dev.staging_bytes = total;
+    }
+    char *base = (char *)g_dev.staging;
+    T *dA = (T *)base, *dB = (T *)(base + a_bytes), *dC = (T *)(base + a_bytes + b_bytes);
+    hipStream_t s = g_dev.stream;
+    const size_t es = sizeof(T);
+    if (need_ab) {
+        UGEMM_HIP_TRY(hipMemcpy2DAsync(dA, h.lda * es, h.A, h.lda * es, a_len * es, a_rows, hipMemcpyHostToDevice, s));
+        UGEMM_HIP_TRY(hipMemcpy2DAsync(dB, h.ldb * es, h.B, h.ldb * es, b_len * es, b_rows, hipMemcpyHostToDevice, s));
+    }
+    if (need_c)
+        UGEMM_HIP_TRY(hipMemcpy2DAsync(dC, h.ldc * es, h.C, h.ldc * es, h.N * es, h.M, hipMemcpyHostToDevice, s));
+    Gemm<T> d = h;
+    d.A = dA;
+    d.B = dB;
+    d.C = dC;
+    int rc;
+    if constexpr (sizeof(T) == 4) rc = run_sgemm(d, mode, -1, s);
+    else rc = run_dgemm(d, -1, s);
+    if (rc) {
+        hipStreamSynchronize(s);
+        return rc;
+    }
+    UGEMM_HIP_TRY(hipMemcpy2DAsync(h.C, h.ldc * es, dC, h.ldc * es, h.N * es, h.M, hipMemcpyDeviceToHost, s));
+    UGEMM_HIP_TRY(hipStreamSynchronize(s));
+    return UGEMM_OK;
+}
+
+static int env_mode() {
+    static int mode = -2;
+    if (mode == -2) {
+        const char *e = getenv("UGEMM_HIP_MODE");
+        mode = UGEMM_MODE_FP32;
+        if (e) {
+            if (!strcmp(e, "bf16x3")) mode = UGEMM_MODE_BF16X3;
+            else if (!strcmp(e, "bf16x1")) mode = UGEMM_MODE_BF16X1;
+            else if (!strcmp(e, "bf16x6")) mode = UGEMM_MODE_BF16X6;
+            else if (!strcmp(e, "fp8")) mode = UGEMM_MODE_FP8_BLOCK;
+        }
+    }
+    return mode;
+}
+
+}  // namespace ugemm
+
+using namespace ugemm;
+
+// ---------------------------------------------------------------------------
+// Exported C ABI.
+extern "C" {
+
+const char *ugemm_hip_version(void) { return "ugemm-hip 0.1 (gfx950)"; }
+
+int ugemm_hip_last_error(void) { return t_status; }
+const char *ugemm_hip_last_error_message(void) { return t_msg; }
+
+const char *ugemm_hip_status_string(int status) {
+    switch (status) {
+        case UGEMM_OK: return "ok";
+        case UGEMM_ERR_ARG: return "invalid argument";
+        case UGEMM_ERR_HIP: return "HIP runtime error";
+        case UGEMM_ERR_NOMEM: return "out of memory";
+        case UGEMM_ERR_NODEV: return "no usable gfx950 device";
+        case UGEMM_ERR_UNSUPPORTED: return "unsupported mode or layout";
+        case UGEMM_ERR_TUNE_FILE: return "tuning cache file error";
+        default: return "unknown status";
+    }
+}
+
+int ugemm_hip_init(int device, size_t max_bytes) {
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    int rc = init_locked(device, max_bytes);
+    return rc ? rc : ok();
+}
+
+void ugemm_hip_finish(void) {
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (!g_dev.ready) return;
+    hipSetDevice(g_dev.device);
+    if (g_dev.stream) hipStreamSynchronize(g_dev.stream);
+    if (g_dev.staging) hipFree(g_dev.staging);
+    if (g_dev.stream) hipStreamDestroy(g_dev.stream);
+    g_dev.staging = nullptr;
+    g_dev.staging_bytes = 0;
+    g_dev.stream = nullptr;
+    g_dev.ready = false;
+}
+
+int ugemm_hip_sgemm(char major, char transa, char transb, int M, int N, int K, float alpha, const float *A, int lda,
+                    const float *B, int ldb, float beta, float *C, int ldc, int mode) {
+    if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    Gemm<float> g;
+    int rc = normalize(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, g);
+    if (rc) return rc;
+    rc = run_host(g, mode);
+    return rc ? rc : ok();
+}
+
+void sgemm_hip(char major, char transa, char transb, int M, int N, int K, float alpha, const float *A, int lda,
+               const float *B, int ldb, float beta, float *C, int ldc) {
+    ugemm_hip_sgemm(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, env_mode());
+}
+
+void dgemm_hip(char major, char transa, char transb, int M, int N, int K, double alpha, const double *A, int lda,
+               const double *B, int ldb, double beta, double *C, int ldc) {
+    Gemm<double> g;
+    int rc = normalize(major, transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, g);
+    if (rc) return;
+    rc = run_host(g, 0);
+    if (!rc) ok();
+}
+
+void sgemm_hip_packed(char ta, char tb, int m, int n, int k, float alpha, float *a, float *b, float beta, float *c) {
+    // Packed row-major operands, as sgemm_ocl (sgemm_ocl2.h:166-219): no
+    // leading dimensions, a 'T' operand is stored transposed and packed.
+    bool tA = !(ta == 'N' || ta == 'n'), tB = !(tb == 'N' || tb == 'n');
+    ugemm_hip_sgemm('R', ta, tb, m, n, k, alpha, a, tA ? m : k, b, tB ? k : n, beta, c, n, env_mode());
+}
+
+int ugemm_hip_sgemm_dev(char major, char transa, char transb, int M, int N, int K, float alpha, const float *dA,
+                        int lda, const float *dB, int ldb, float beta, float *dC, int ldc, int mode, void *stream) {
+    if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    int rc = ensure_device();
+    if (rc) return rc;
+    Gemm<float> g;
+    rc = normalize(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, g);
+    if (rc) return rc;
+    rc = run_sgemm(g, mode, -1, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int K, double alpha, const double *dA,
+                        int lda, const double *dB, int ldb, double beta, double *dC, int ldc, void *stream) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    Gemm<double> g;
+    rc = normalize(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, g);
+    if (rc) return rc;
+    rc = run_dgemm(g, -1, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+int ugemm_hip_variant_count(int mode) {
+    switch (mode) {
+        case UGEMM_MODE_FP32: return fp32_variant_count();
+        case UGEMM_MODE_FP8_BLOCK: return fp8_variant_count();
+        case -1: return dgemm_variant_count();
+        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: return mfma_variant_count(mode);
+        default: return 0;
+    }
+}
+
+const char *ugemm_hip_variant_name(int mode, int v) {
+    switch (mode) {
+        case UGEMM_MODE_FP32: return fp32_variant_name(v);
+        case UGEMM_MODE_FP8_BLOCK: return fp8_variant_name(v);
+        case -1: return dgemm_variant_name(v);
+        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: return mfma_variant_name(mode, v);
+        default: return nullptr;
+    }
+}
+
+}  // extern "C"
