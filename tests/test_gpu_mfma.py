@@ -1,0 +1,113 @@
+"""Parity of the split-BF16 MFMA forms (BF16X1, BF16X3, BF16X6) with the oracle.
+
+Acceptance per element: |C - C_exact| <= (2(K+1) eps32 + extra) * scale with
+extra = 2^-6 (x1), 2^-14 (x3), 2^-21 (x6) -- the per-product error of the
+dropped split terms (see tests/helpers.py)."""
+import zlib
+
+import numpy as np
+import pytest
+
+import ugemm_amd as ug
+from ugemm_amd import hip
+from oracle import oracle as orc
+from tests.helpers import Case
+
+pytestmark = pytest.mark.gpu
+
+MODES = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6]
+COMBOS = [(m, a, b) for m in "RC" for a in "NT" for b in "NT"]
+SHAPES = [(1, 1, 1), (7, 5, 9), (33, 65, 31), (129, 127, 65), (200, 300, 257)]
+
+
+def run_host(case, mode):
+    C = case.C0.copy()
+    ug.sgemm(case.major, case.ta, case.tb, case.M, case.N, case.K, case.alpha, case.A, case.lda, case.B, case.ldb,
+             case.beta, C, case.ldc, mode=mode)
+    return C
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("major,ta,tb", COMBOS)
+def test_split_layouts_shapes(mode, major, ta, tb):
+    for (M, N, K) in SHAPES:
+        seed = zlib.crc32(f"{mode}{major}{ta}{tb}{M},{N},{K}".encode())
+        case = Case(major, ta, tb, M, N, K, 1.5, 0.5, seed=seed)
+        C = run_host(case, mode)
+        ok, worst = case.judge(C, mode=mode)
+        assert ok, f"mode {mode} {major}{ta}{tb} {M}x{N}x{K}: worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_split_every_variant(mode, variant, ta, tb):
+    ug.force_variant(mode, variant)
+    try:
+        for (M, N, K) in ((130, 250, 70), (300, 260, 96)):
+            case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=31 + variant)
+            C = run_host(case, mode)
+            ok, worst = case.judge(C, mode=mode)
+            assert ok, f"variant {variant} {M}x{N}x{K}: {worst:.3g}"
+    finally:
+        ug.force_variant(mode, -1)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("pad", [1, 4])
+def test_split_leading_dimensions_and_nan_c(mode, pad):
+    case = Case("R", "T", "T", 131, 77, 93, 1.5, 0.5, seed=11 + pad, pad=pad)
+    C = run_host(case, mode)
+    assert case.judge(C, mode=mode)[0]
+    assert case.untouched_padding(C)
+    case = Case("R", "N", "N", 70, 90, 50, 1.0, 0.0, seed=3)
+    case.C0[:] = np.nan
+    C = run_host(case, mode)
+    assert np.all(np.isfinite(C))
+
+
+def test_split_accuracy_ordering():
+    # K = 4096, U[-1,1]: rms error vs exact -- bf16x6 at or below plain FP32,
+    # bf16x3 within a small factor of it, bf16x1 far above (8-bit operands).
+    M = N = 256
+    K = 4096
+    case = Case("R", "N", "N", M, N, K, 1.0, 0.0, seed=77)
+    exact, _ = case.exact()
+    rms = {}
+    for mode in (ug.MODE_FP32, ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6):
+        C = run_host(case, mode)
+        rms[mode] = float(np.sqrt(np.mean((C.astype(np.float64) - exact) ** 2)))
+    assert rms[ug.MODE_BF16X6] <= rms[ug.MODE_FP32], rms
+    assert rms[ug.MODE_BF16X3] <= 4 * rms[ug.MODE_FP32], rms
+    assert rms[ug.MODE_BF16X1] > 100 * rms[ug.MODE_FP32], rms
+
+
+def test_split_special_values():
+    # An infinite operand: BF16X1 keeps +inf; the split forms multiply the
+    # infinite leading plane by the other operand's trailing planes, which are
+    # 0 for bf16-exact values, so inf * 0 makes those entries NaN (documented
+    # behaviour of split GEMMs). Rows without the inf stay exact.
+    M, N, K = 16, 16, 32
+    A = np.ones((M, K), np.float32)
+    B = np.ones((K, N), np.float32)
+    A[3, 5] = np.inf
+    for mode in MODES:
+        C = np.zeros((M, N), np.float32)
+        ug.sgemm("R", "N", "N", M, N, K, 1.0, A, K, B, N, 0.0, C, N, mode=mode)
+        if mode == ug.MODE_BF16X1:
+            assert np.all(np.isposinf(C[3])), mode
+        else:
+            assert not np.any(np.isfinite(C[3])), mode
+        assert np.all(C[np.arange(M) != 3] == K), mode
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_split_large_sampled(mode):
+    M = N = K = 2048
+    case = Case("R", "N", "N", M, N, K, 1.5, 0.5, seed=51)
+    C = run_host(case, mode)
+    rows = np.sort(orc.rng(52).choice(M, 48, replace=False))
+    exact, scale = case.exact(rows=rows)
+    from tests.helpers import MODE_EXTRA
+    ok, worst = orc.check_bound(C[rows, :N], exact, scale, orc.tolerance(K, extra=MODE_EXTRA[mode]))
+    assert ok, f"worst/bound {worst:.3g}"

@@ -1,10 +1,490 @@
-// Split-BF16 SGEMM on MFMA (placeholder until the kernels land).
+// Split-BF16 SGEMM on gfx950 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulate.
+//
+// Splitting: every FP32 operand x is written as
+//   x = h0 + h1 + h2 + r,  h0 = bf16(x), h1 = bf16(x - h0), h2 = bf16(x - h0 - h1)
+// (round-to-nearest-even; the differences are exact in FP32).  NP planes are
+// kept and the products
+//   NP = 1 (BF16X1): h0*h0
+//   NP = 2 (BF16X3): h0*h0 + h0*h1 + h1*h0                  |error| <~ 3*2^-16 |a||b|
+//   NP = 3 (BF16X6): + h1*h1 + h0*h2 + h2*h0                |error| <~ 6*2^-24 |a||b|
+// are issued as MFMAs into one FP32 accumulator, smallest terms first.
+// Products of bf16 values are exact in FP32, so the remaining error is the
+// dropped terms plus FP32 accumulation.
+//
+// Two kernels per call:
+//  1. split_prep (HBM-bound): reads op(A) / op(B) once in any layout and
+//     writes NP bf16 planes in "NT" form -- rows of A (m) and of B (n), k
+//     contiguous -- zero-padded to whole tiles, so that the core needs no
+//     bounds checks and no layout variants.  Transposed sources go through
+//     an LDS tile so both the read and the write are coalesced.
+//  2. bf16_nt_core (MFMA-bound): one workgroup per BM x BN tile of C, 4 waves
+//     (2 x 2), each wave a (BM/2) x (BN/2) grid of 16x16 fragments whose
+//     FP32 accumulators live in AGPRs; per K step of BK, all NP planes of the
+//     A and B tiles are staged global -> registers -> LDS (two stages, the
+//     next one in flight during the MFMAs), and every fragment read from LDS
+//     serves every product that uses its plane.
+// Fragment layout (16x16x32 bf16): lane l holds row l%16, k = 8*(l/16)..+7
+// of A and the same k of column l%16 of B; D holds rows 4*(l/16)+i, column
+// l%16.  A and B use the same k assignment, so any order of the hardware's
+// internal sum pairs the right elements.
 #include "internal.h"
+#include "kernel_common.h"
+
 namespace ugemm {
-int mfma_variant_count(int) { return 0; }
-const char *mfma_variant_name(int, int) { return nullptr; }
-int mfma_pick_variant(const Gemm<float> &, int) { return 0; }
-int launch_mfma(const Gemm<float> &, int mode, int, hipStream_t) {
-    return fail(UGEMM_ERR_UNSUPPORTED, "mode %d not built yet", mode);
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Splitting.
+template <int NP, int W>
+__device__ __forceinline__ void split_vec(const float (&x)[W], __bf16 (&out)[NP][W]) {
+    typedef float fv __attribute__((ext_vector_type(W)));
+    typedef __bf16 bv __attribute__((ext_vector_type(W)));
+    fv v;
+#pragma unroll
+    for (int i = 0; i < W; ++i) v[i] = x[i];
+    bv h = __builtin_convertvector(v, bv);
+#pragma unroll
+    for (int i = 0; i < W; ++i) out[0][i] = h[i];
+    if constexpr (NP > 1) {
+        fv hf = __builtin_convertvector(h, fv);
+        fv r;
+#pragma unroll
+        for (int i = 0; i < W; ++i) r[i] = __builtin_isfinite(hf[i]) ? v[i] - hf[i] : 0.0f;
+        bv l = __builtin_convertvector(r, bv);
+#pragma unroll
+        for (int i = 0; i < W; ++i) out[1][i] = l[i];
+        if constexpr (NP > 2) {
+            fv lf = __builtin_convertvector(l, fv);
+            fv r2 = r - lf;
+            bv l2 = __builtin_convertvector(r2, bv);
+#pragma unroll
+            for (int i = 0; i < W; ++i) out[2][i] = l2[i];
+        }
+    }
 }
+
+template <int NP>
+__device__ __forceinline__ void write_planes(__bf16 *dst, long plane_stride, const float (&x)[8]) {
+    __bf16 o[NP][8];
+    split_vec<NP, 8>(x, o);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        bf16x8 w;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) w[j] = o[p][j];
+        *reinterpret_cast<bf16x8 *>(dst + p * plane_stride) = w;
+    }
+}
+
+// Source element (r, k) of the operand: KCONT ? X[r*ld + k] : X[k*ld + r].
+// Output plane p: out[p*plane_stride + r*ldo + k] for r < rows_pad, k < k_pad
+// (zeros outside rows x K).  KCONT version: one thread per 8 k of one row.
+template <int NP>
+__global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                        bool vec, __bf16 *__restrict__ out, long plane_stride,
+                                                        long ldo, int rows_pad, int k_pad) {
+    const long chunks_per_row = k_pad / 8;
+    const long total = (long)rows_pad * chunks_per_row;
+    for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(f / chunks_per_row);
+        const int k = (int)(f - (long)r * chunks_per_row) * 8;
+        float x[8];
+        const float *p = X + (long)r * ld + k;
+        if (r < rows && k + 8 <= K && vec) {
+            const float4 a = *reinterpret_cast<const float4 *>(p);
+            const float4 b = *reinterpret_cast<const float4 *>(p + 4);
+            x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = (r < rows && k + j < K) ? p[j] : 0.0f;
+        }
+        write_planes<NP>(out + (long)r * ldo + k, plane_stride, x);
+    }
+}
+
+// Row-contiguous source (X[k*ld + r]): 64 x 64 tiles transposed through LDS.
+template <int NP>
+__global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                        bool vec, __bf16 *__restrict__ out, long plane_stride,
+                                                        long ldo, int rows_pad, int k_pad) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    // Read: 64 k-lines of 64 r each, as 16 float4 per line (coalesced along r).
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int f = t + i * 256;
+        const int kk = f / 16, rr = (f % 16) * 4;
+        const int k = k0 + kk, r = r0 + rr;
+        const float *p = X + (long)k * ld + r;
+        float v[4];
+        if (k < K && r + 4 <= rows && vec) {
+            const float4 a = *reinterpret_cast<const float4 *>(p);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = (k < K && r + j < rows) ? p[j] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tile[kk][rr + j] = v[j];
+    }
+    __syncthreads();
+    // Write: 64 rows of 8 chunks of 8 k.
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int f = t + i * 256;
+        const int rr = f / 8, kk = (f % 8) * 8;
+        const int r = r0 + rr, k = k0 + kk;
+        if (r >= rows_pad || k >= k_pad) continue;
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = tile[kk + j][rr];
+        write_planes<NP>(out + (long)r * ldo + k, plane_stride, x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA core over prepared planes.
+
+// 16-byte chunk swizzle inside a plane row of CPR chunks: conflict-free
+// fragment reads (16 rows, one chunk each) and row writes for both 32- and
+// 64-bank LDS layouts (checked exhaustively for rows in 16-row windows).
+template <int CPR>
+__device__ __forceinline__ int swz_chunk(int row, int c) {
+    if constexpr (CPR == 8) return c ^ ((row ^ (row >> 3)) & 7);
+    else return c ^ (((row >> 1) ^ (row >> 3)) & 3);
+}
+
+template <int NP>
+struct Products;
+template <>
+struct Products<1> {
+    static constexpr int N = 1;
+    static constexpr int a[1] = {0}, b[1] = {0};
+};
+template <>
+struct Products<2> {
+    static constexpr int N = 3;
+    static constexpr int a[3] = {1, 0, 0}, b[3] = {0, 1, 0};
+};
+template <>
+struct Products<3> {
+    static constexpr int N = 6;
+    static constexpr int a[6] = {2, 0, 1, 1, 0, 0}, b[6] = {0, 2, 1, 0, 1, 0};
+};
+
+template <int BM_, int BN_, int BK_, int NP_>
+struct CoreCfg {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, NP = NP_;
+    static constexpr int WGM = 2, WGN = 2, NT = 256;
+    static constexpr int WTM = BM / WGM, WTN = BN / WGN;
+    static constexpr int FM = WTM / 16, FN = WTN / 16;
+    static constexpr int KSTEPS = BK / 32, CPR = BK / 8;
+    static constexpr int A_PLANE = BM * BK, B_PLANE = BN * BK;           // bf16 elements
+    static constexpr int STAGE = NP * (A_PLANE + B_PLANE);
+    static constexpr int STAGES = (2 * STAGE * 2 <= 160 * 1024) ? 2 : 1;
+    static constexpr int LDS_BYTES = STAGES * STAGE * 2;
+    static constexpr int CHUNKS = NP * (BM + BN) * CPR;                 // 16-byte chunks per stage
+    static constexpr int PER_T = CHUNKS / NT;
+    static_assert(CHUNKS % NT == 0, "stage must split evenly over the threads");
+    static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 32 == 0, "tile shape");
+};
+
+template <class CF>
+__global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, const __bf16 *__restrict__ Ap,
+                                                       long a_plane, long lda, const __bf16 *__restrict__ Bp,
+                                                       long b_plane, long ldb, float alpha, float beta,
+                                                       float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
+    constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+    __bf16 *lds = reinterpret_cast<__bf16 *>(smem_raw);
+
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w / CF::WGN, wn = w % CF::WGN;
+
+    // Stage copy: chunk i of thread t is f = t + NT*i, A chunks first.  With
+    // rows of CPR 16-byte chunks, chunk i of A covers plane (R / BM) and row
+    // (R % BM) with R = t/CPR + (NT/CPR)*i, and chunk column t%CPR; the plane
+    // and the row offset are the same for all threads of a given i, so only
+    // one base address per operand lives in VGPRs.
+    constexpr int RSTEP = CF::NT / CPR;                  // rows advanced per chunk index
+    constexpr int A_CH = NP * BM * CPR / CF::NT;         // chunks per thread from A
+    static_assert((NP * BM * CPR) % CF::NT == 0 && BM % RSTEP == 0 && BN % RSTEP == 0, "stage split");
+    const int trow = t / CPR, tc = t % CPR;
+    const __bf16 *a_src = Ap + (long)(m0 + trow) * lda + tc * 8;
+    const __bf16 *b_src = Bp + (long)(n0 + trow) * ldb + tc * 8;
+    bf16x8 stage_regs[PER_T];
+    auto load_stage = [&](int kt) {
+        const long k0 = (long)kt * BK;
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) {
+            if (i < A_CH) {
+                const int R = RSTEP * i, p = R / BM, r = R % BM;
+                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(a_src + p * a_plane + r * lda + k0);
+            } else {
+                const int R = RSTEP * (i - A_CH), p = R / BN, r = R % BN;
+                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(b_src + p * b_plane + r * ldb + k0);
+            }
+        }
+    };
+    auto store_stage = [&](int buf) {
+        __bf16 *base = lds + buf * STAGE;
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) {
+            if (i < A_CH) {
+                const int R = RSTEP * i, p = R / BM, row = trow + R % BM;
+                *reinterpret_cast<bf16x8 *>(base + p * CF::A_PLANE + row * BK + swz_chunk<CPR>(row, tc) * 8) =
+                    stage_regs[i];
+            } else {
+                const int R = RSTEP * (i - A_CH), p = R / BN, row = trow + R % BN;
+                *reinterpret_cast<bf16x8 *>(base + NP * CF::A_PLANE + p * CF::B_PLANE + row * BK +
+                                            swz_chunk<CPR>(row, tc) * 8) = stage_regs[i];
+            }
+        }
+    };
+
+    floatx4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int ktiles = Kp / BK;
+    load_stage(0);
+    store_stage(0);
+    __syncthreads();
+
+    const int frow = lane & 15, fc = lane >> 4;
+    for (int kt = 0; kt < ktiles; ++kt) {
+        const int buf = (STAGES == 2) ? (kt & 1) : 0;
+        const bool more = kt + 1 < ktiles;
+        if (more) load_stage(kt + 1);
+        const __bf16 *as = lds + buf * STAGE;
+        const __bf16 *bs = as + NP * CF::A_PLANE;
+#pragma unroll
+        for (int s = 0; s < CF::KSTEPS; ++s) {
+            bf16x8 fa[NP][FM], fb[NP][FN];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    const int row = wm * CF::WTM + i * 16 + frow;
+                    fa[p][i] = *reinterpret_cast<const bf16x8 *>(as + p * CF::A_PLANE + row * BK +
+                                                                 swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                }
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const int row = wn * CF::WTN + j * 16 + frow;
+                    fb[p][j] = *reinterpret_cast<const bf16x8 *>(bs + p * CF::B_PLANE + row * BK +
+                                                                 swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < Products<NP>::N; ++q)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            fa[Products<NP>::a[q]][i], fb[Products<NP>::b[q]][j], acc[i][j], 0, 0, 0);
+        }
+        if (STAGES == 1) __syncthreads();
+        if (more) store_stage(STAGES == 2 ? (buf ^ 1) : 0);
+        __syncthreads();
+    }
+
+    // Epilogue: C = alpha * acc (+ beta * C), rows 4*(l/16)+r, column l%16 of each fragment.
+    const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4 + r;
+                if (interior || (m < M && n < N)) {
+                    float *cp = C + (long)m * ldc + n;
+                    const float v = alpha * acc[i][j][r];
+                    *cp = (beta == 0.0f) ? v : fmaf(alpha, acc[i][j][r], beta * *cp);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variants: (tile, BK) per plane count.
+struct VariantInfo {
+    const char *name;
+    int bm, bn, bk;
+};
+
+template <int NP>
+struct Table;
+// NP = 1: 256x256x32 (32 KiB/stage); NP = 2: 256x256x32 (64 KiB/stage);
+// NP = 3: 256x128x32 (72 KiB/stage).  Smaller tiles serve small grids.
+template <>
+struct Table<1> {
+    using V0 = CoreCfg<256, 256, 32, 1>;
+    using V1 = CoreCfg<128, 128, 32, 1>;
+    using V2 = CoreCfg<256, 128, 32, 1>;
+    using V3 = CoreCfg<64, 64, 32, 1>;
+};
+template <>
+struct Table<2> {
+    using V0 = CoreCfg<256, 256, 32, 2>;
+    using V1 = CoreCfg<128, 128, 32, 2>;
+    using V2 = CoreCfg<256, 128, 32, 2>;
+    using V3 = CoreCfg<64, 64, 32, 2>;
+};
+template <>
+struct Table<3> {
+    using V0 = CoreCfg<256, 128, 32, 3>;
+    using V1 = CoreCfg<128, 128, 32, 3>;
+    using V2 = CoreCfg<128, 256, 32, 3>;
+    using V3 = CoreCfg<64, 64, 32, 3>;
+};
+constexpr int kNumVariants = 4;
+
+template <class CF>
+const char *cfg_name() {
+    static char buf[48];
+    snprintf(buf, sizeof buf, "bf16x%d_nt_%dx%dx%d", CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN, CF::BK);
+    return buf;
+}
+
+int planes_of(int mode) {
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: return 1;
+        case UGEMM_MODE_BF16X3: return 2;
+        case UGEMM_MODE_BF16X6: return 3;
+        default: return 0;
+    }
+}
+
+static long round_up(long x, long a) { return (x + a - 1) / a * a; }
+
+template <int NP>
+int prep_operand(const float *X, long ld, bool kcont, int rows, int K, __bf16 *out, long plane_stride, long ldo,
+                 int rows_pad, int k_pad, hipStream_t s) {
+    const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
+    if (kcont) {
+        const long total = (long)rows_pad * (k_pad / 8);
+        long blocks = (total + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(split_prep_kcont<NP>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, vec, out,
+                           plane_stride, ldo, rows_pad, k_pad);
+    } else {
+        dim3 grid((unsigned)((rows_pad + 63) / 64), (unsigned)((k_pad + 63) / 64));
+        hipLaunchKernelGGL(split_prep_rcont<NP>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, plane_stride, ldo,
+                           rows_pad, k_pad);
+    }
+    UGEMM_HIP_TRY(hipGetLastError());
+    return UGEMM_OK;
+}
+
+template <class CF>
+int run_cfg(const Gemm<float> &g, hipStream_t s) {
+    constexpr int NP = CF::NP;
+    const long Mp = round_up(g.M, CF::BM), Np = round_up(g.N, CF::BN), Kp = round_up(g.K, 64);
+    const long a_plane = Mp * Kp, b_plane = Np * Kp;
+    const size_t bytes = (size_t)NP * (a_plane + b_plane) * sizeof(__bf16);
+    const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
+    if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
+    void *ws = nullptr;
+    int rc = stream_alloc(&ws, bytes, s);
+    if (rc) return rc;
+    __bf16 *Ap = (__bf16 *)ws, *Bp = Ap + NP * a_plane;
+    rc = prep_operand<NP>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, Kp, (int)Mp, (int)Kp, s);
+    if (!rc) rc = prep_operand<NP>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, Kp, (int)Np, (int)Kp, s);
+    if (!rc) {
+        auto kern = bf16_nt_core<CF>;
+        static bool attr = false;
+        if (!attr) {
+            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               CF::LDS_BYTES);
+            if (e != hipSuccess) rc = hip_fail(e, "hipFuncSetAttribute");
+            attr = true;
+        }
+        if (!rc) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
+                               (int)Kp, Ap, a_plane, Kp, Bp, b_plane, Kp, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = hip_fail(e, "bf16_nt_core launch");
+        }
+    }
+    int rc2 = stream_free(ws, s);
+    return rc ? rc : rc2;
+}
+
+template <int NP>
+int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
+    using T = Table<NP>;
+    switch (variant) {
+        case 0: return run_cfg<typename T::V0>(g, s);
+        case 1: return run_cfg<typename T::V1>(g, s);
+        case 2: return run_cfg<typename T::V2>(g, s);
+        case 3: return run_cfg<typename T::V3>(g, s);
+        default: return fail(UGEMM_ERR_ARG, "split-bf16: no variant %d", variant);
+    }
+}
+
+template <int NP>
+const char *name_np(int v) {
+    using T = Table<NP>;
+    static const char *names[kNumVariants] = {nullptr};
+    if (!names[0]) {
+        static char b0[48], b1[48], b2[48], b3[48];
+        snprintf(b0, 48, "%s", cfg_name<typename T::V0>());
+        snprintf(b1, 48, "%s", cfg_name<typename T::V1>());
+        snprintf(b2, 48, "%s", cfg_name<typename T::V2>());
+        snprintf(b3, 48, "%s", cfg_name<typename T::V3>());
+        names[1] = b1; names[2] = b2; names[3] = b3; names[0] = b0;
+    }
+    return names[v];
+}
+
+}  // namespace
+
+int mfma_variant_count(int mode) { return planes_of(mode) ? kNumVariants : 0; }
+
+const char *mfma_variant_name(int mode, int v) {
+    if (v < 0 || v >= kNumVariants) return nullptr;
+    switch (planes_of(mode)) {
+        case 1: return name_np<1>(v);
+        case 2: return name_np<2>(v);
+        case 3: return name_np<3>(v);
+        default: return nullptr;
+    }
+}
+
+int mfma_pick_variant(const Gemm<float> &g, int mode) {
+    const long cus = device_cu_count();
+    const int np = planes_of(mode);
+    // Largest tile whose grid still gives every CU work.
+    const long big_m = 256, big_n = (np == 3) ? 128 : 256;
+    const long tiles_big = ((g.M + big_m - 1) / big_m) * ((g.N + big_n - 1) / big_n);
+    if (tiles_big >= cus) return 0;
+    const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+    if (t128 >= cus / 2) return 1;
+    return 3;
+}
+
+int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
+    switch (planes_of(mode)) {
+        case 1: return launch_np<1>(g, variant, s);
+        case 2: return launch_np<2>(g, variant, s);
+        case 3: return launch_np<3>(g, variant, s);
+        default: return fail(UGEMM_ERR_UNSUPPORTED, "mode %d is not a split-bf16 form", mode);
+    }
+}
+
 }  // namespace ugemm

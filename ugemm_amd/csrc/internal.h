@@ -64,10 +64,11 @@ int forced_variant(int mode);                                         // -1 if n
 void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms);
 
 // ---- device scratch (runtime.hip) ----
-// Returns a device buffer of at least `bytes` bytes, owned by the runtime and
-// reused across calls on the calling stream's device. Not for host-pointer
-// staging (see staging_buffer).
-int workspace(size_t bytes, void **ptr);
+// Stream-ordered workspace for the operand-preparation kernels: allocated
+// from the device's default memory pool on stream s (which keeps freed
+// blocks for reuse), released in stream order after the consuming kernel.
+int stream_alloc(void **ptr, size_t bytes, hipStream_t s);
+int stream_free(void *ptr, hipStream_t s);
 
 }  // namespace ugemm
 

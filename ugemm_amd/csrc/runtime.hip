@@ -76,6 +76,14 @@ static int init_locked(int device, size_t max_bytes) {
         g_dev.device = device;
         g_dev.cus = prop.multiProcessorCount;
         UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        // Keep freed workspace blocks in the default pool (no release at
+        // synchronisation points), so that repeated calls do not go back to
+        // the driver for memory.
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
         g_dev.ready = true;
     }
     if (max_bytes > g_dev.staging_bytes) {
@@ -98,10 +106,18 @@ int ensure_device() {
 
 int device_cu_count() { return g_dev.ready ? g_dev.cus : 256; }
 
-int workspace(size_t bytes, void **ptr) {
-    // Stream-ordered allocations are made by the callers with hipMallocAsync;
-    // this synchronous fallback is for set-up paths only.
-    UGEMM_HIP_TRY(hipMalloc(ptr, bytes));
+int stream_alloc(void **ptr, size_t bytes, hipStream_t s) {
+    *ptr = nullptr;
+    if (bytes == 0) return UGEMM_OK;
+    hipError_t e = hipMallocAsync(ptr, bytes, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(workspace)");
+    return UGEMM_OK;
+}
+
+int stream_free(void *ptr, hipStream_t s) {
+    if (!ptr) return UGEMM_OK;
+    hipError_t e = hipFreeAsync(ptr, s);
+    if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(workspace)");
     return UGEMM_OK;
 }
 

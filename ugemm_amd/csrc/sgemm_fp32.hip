@@ -13,31 +13,12 @@
 // Semantics follow the reference's row-major kernel gemm_rnn (sgemm_ocl2.h:17-90):
 // C = alpha * acc + beta * C, C is not read when beta == 0 (sgemm_ocl2.h:86-88).
 #include "internal.h"
-#include <stdio.h>
+#include "kernel_common.h"
 
 namespace ugemm {
 namespace {
 
 constexpr int LDS_PAD = 4;   // floats; keeps rows 16-byte aligned
-
-// Maps the launch's linear block id to a (tile_m, tile_n) pair.
-// 1) XCD de-interleave: consecutive hardware block ids go round-robin to the
-//    8 XCDs; renumber so that each XCD owns a contiguous range of ids.
-// 2) Grouped row-major order: GROUP_M tile-rows are walked column by column so
-//    that concurrently resident blocks share A row-panels and B column-panels.
-__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
-    const int total = tiles_m * tiles_n;
-    int pid = bid;
-    if ((total & 7) == 0) pid = (bid & 7) * (total >> 3) + (bid >> 3);
-    constexpr int GROUP_M = 8;
-    const int per_group = GROUP_M * tiles_n;
-    const int group = pid / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(tiles_m - first_m, GROUP_M);
-    const int in_group = pid - group * per_group;
-    tm = first_m + in_group % gsize;
-    tn = in_group / gsize;
-}
 
 // Loads one operand tile (a C_EXT x S_EXT block: C_EXT elements along the
 // operand's contiguous dimension, S_EXT along its strided dimension) from
