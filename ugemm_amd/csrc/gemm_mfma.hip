@@ -79,13 +79,23 @@ __device__ __forceinline__ void write_planes(__bf16 *dst, long plane_stride, con
     }
 }
 
-// Source element (r, k) of the operand: KCONT ? X[r*ld + k] : X[k*ld + r].
-// Output plane p: out[p*plane_stride + r*ldo + k] for r < rows_pad, k < k_pad
-// (zeros outside rows x K).  KCONT version: one thread per 8 k of one row.
+// Plane layout written by the preparation kernels ("k-tile blocked"): plane
+// p of an operand with rows_pad rows and k_pad (multiple of 32) columns holds
+// element (r, k) at
+//   p * plane_stride + ((k / 32) * rows_pad + r) * 32 + (chunk ^ sw(r)) * 8 + k % 8,
+// chunk = (k % 32) / 8, sw(r) = (r >> 1) & 3.  One K tile of BM rows is thus a
+// contiguous BM * 64-byte block whose bytes are already in the (bank-conflict
+// free, see swz_chunk) order of the LDS stage: the core copies it verbatim.
+__device__ __forceinline__ long blocked_offset(int r, int k, int rows_pad) {
+    return ((long)(k >> 5) * rows_pad + r) * 32 + ((((k >> 3) & 3) ^ ((r >> 1) & 3)) << 3);
+}
+
+// Source element (r, k) of the operand: KCONT ? X[r*ld + k] : X[k*ld + r];
+// zeros outside rows x K.  KCONT version: one thread per 8 k of one row.
 template <int NP>
 __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict__ X, long ld, int rows, int K,
                                                         bool vec, __bf16 *__restrict__ out, long plane_stride,
-                                                        long ldo, int rows_pad, int k_pad) {
+                                                        int rows_pad, int k_pad) {
     const long chunks_per_row = k_pad / 8;
     const long total = (long)rows_pad * chunks_per_row;
     for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x) {
@@ -101,7 +111,7 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
 #pragma unroll
             for (int j = 0; j < 8; ++j) x[j] = (r < rows && k + j < K) ? p[j] : 0.0f;
         }
-        write_planes<NP>(out + (long)r * ldo + k, plane_stride, x);
+        write_planes<NP>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
     }
 }
 
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
 template <int NP>
 __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict__ X, long ld, int rows, int K,
                                                         bool vec, __bf16 *__restrict__ out, long plane_stride,
-                                                        long ldo, int rows_pad, int k_pad) {
+                                                        int rows_pad, int k_pad) {
     __shared__ float tile[64][65];
     const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
     const int t = threadIdx.x;
@@ -142,20 +152,21 @@ __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict_
         float x[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = tile[kk + j][rr];
-        write_planes<NP>(out + (long)r * ldo + k, plane_stride, x);
+        write_planes<NP>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
     }
 }
 
 // ---------------------------------------------------------------------------
 // MFMA core over prepared planes.
 
-// 16-byte chunk swizzle inside a plane row of CPR chunks: conflict-free
-// fragment reads (16 rows, one chunk each) and row writes for both 32- and
-// 64-bank LDS layouts (checked exhaustively for rows in 16-row windows).
+// 16-byte chunk swizzle inside a plane row of CPR chunks: chunk c of row r is
+// stored at c ^ sw(r), sw(r) = (r >> 1) & 3 (CPR 4) or r & 7 (CPR 8).
+// Measured with SQ_LDS_BANK_CONFLICT on gfx950: fragment reads (16 lanes =
+// 16 consecutive rows, one chunk each) are conflict-free with these.
 template <int CPR>
 __device__ __forceinline__ int swz_chunk(int row, int c) {
-    if constexpr (CPR == 8) return c ^ ((row ^ (row >> 3)) & 7);
-    else return c ^ (((row >> 1) ^ (row >> 3)) & 3);
+    if constexpr (CPR == 8) return c ^ (row & 7);
+    else return c ^ ((row >> 1) & 3);
 }
 
 template <int NP>
@@ -195,8 +206,8 @@ struct CoreCfg {
 
 template <class CF>
 __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, const __bf16 *__restrict__ Ap,
-                                                       long a_plane, long lda, const __bf16 *__restrict__ Bp,
-                                                       long b_plane, long ldb, float alpha, float beta,
+                                                       long a_plane, int Mp, const __bf16 *__restrict__ Bp,
+                                                       long b_plane, int Np, float alpha, float beta,
                                                        float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
     constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
@@ -209,45 +220,34 @@ __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, con
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w / CF::WGN, wn = w % CF::WGN;
 
-    // Stage copy: chunk i of thread t is f = t + NT*i, A chunks first.  With
-    // rows of CPR 16-byte chunks, chunk i of A covers plane (R / BM) and row
-    // (R % BM) with R = t/CPR + (NT/CPR)*i, and chunk column t%CPR; the plane
-    // and the row offset are the same for all threads of a given i, so only
-    // one base address per operand lives in VGPRs.
-    constexpr int RSTEP = CF::NT / CPR;                  // rows advanced per chunk index
-    constexpr int A_CH = NP * BM * CPR / CF::NT;         // chunks per thread from A
-    static_assert((NP * BM * CPR) % CF::NT == 0 && BM % RSTEP == 0 && BN % RSTEP == 0, "stage split");
-    const int trow = t / CPR, tc = t % CPR;
-    const __bf16 *a_src = Ap + (long)(m0 + trow) * lda + tc * 8;
-    const __bf16 *b_src = Bp + (long)(n0 + trow) * ldb + tc * 8;
+    // Stage copy.  In the k-tile blocked plane layout the A tile of K tile kt
+    // and plane p is the contiguous block Ap + p*a_plane + (kt*Mp + m0)*BK of
+    // BM*BK elements, already in LDS order; likewise B.  Chunk i of thread t
+    // is f = t + NT*i (A chunks first), copied to the same position in LDS.
+    static_assert(BK == 32, "blocked plane layout uses 32-wide K tiles");
+    constexpr int A_CH = NP * BM * CPR / CF::NT;         // 16-byte chunks per thread from A
+    static_assert((NP * BM * CPR) % CF::NT == 0 && (BM * CPR) % CF::NT == 0 && (BN * CPR) % CF::NT == 0,
+                  "stage split");
+    const __bf16 *a_src = Ap + (long)m0 * BK + t * 8;
+    const __bf16 *b_src = Bp + (long)n0 * BK + t * 8;
     bf16x8 stage_regs[PER_T];
     auto load_stage = [&](int kt) {
-        const long k0 = (long)kt * BK;
+        const long a_k = (long)kt * Mp * BK, b_k = (long)kt * Np * BK;
 #pragma unroll
         for (int i = 0; i < PER_T; ++i) {
             if (i < A_CH) {
-                const int R = RSTEP * i, p = R / BM, r = R % BM;
-                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(a_src + p * a_plane + r * lda + k0);
+                const int e = i * CF::NT * 8, p = e / (BM * BK), off = e % (BM * BK);
+                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(a_src + p * a_plane + a_k + off);
             } else {
-                const int R = RSTEP * (i - A_CH), p = R / BN, r = R % BN;
-                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(b_src + p * b_plane + r * ldb + k0);
+                const int e = (i - A_CH) * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
+                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(b_src + p * b_plane + b_k + off);
             }
         }
     };
     auto store_stage = [&](int buf) {
-        __bf16 *base = lds + buf * STAGE;
+        __bf16 *base = lds + buf * STAGE + t * 8;
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) {
-            if (i < A_CH) {
-                const int R = RSTEP * i, p = R / BM, row = trow + R % BM;
-                *reinterpret_cast<bf16x8 *>(base + p * CF::A_PLANE + row * BK + swz_chunk<CPR>(row, tc) * 8) =
-                    stage_regs[i];
-            } else {
-                const int R = RSTEP * (i - A_CH), p = R / BN, row = trow + R % BN;
-                *reinterpret_cast<bf16x8 *>(base + NP * CF::A_PLANE + p * CF::B_PLANE + row * BK +
-                                            swz_chunk<CPR>(row, tc) * 8) = stage_regs[i];
-            }
-        }
+        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<bf16x8 *>(base + i * CF::NT * 8) = stage_regs[i];
     };
 
     floatx4 acc[FM][FN];
@@ -373,18 +373,18 @@ int planes_of(int mode) {
 static long round_up(long x, long a) { return (x + a - 1) / a * a; }
 
 template <int NP>
-int prep_operand(const float *X, long ld, bool kcont, int rows, int K, __bf16 *out, long plane_stride, long ldo,
-                 int rows_pad, int k_pad, hipStream_t s) {
+int prep_operand(const float *X, long ld, bool kcont, int rows, int K, __bf16 *out, long plane_stride, int rows_pad,
+                 int k_pad, hipStream_t s) {
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
     if (kcont) {
         const long total = (long)rows_pad * (k_pad / 8);
         long blocks = (total + 255) / 256;
         if (blocks > 8192) blocks = 8192;
         hipLaunchKernelGGL(split_prep_kcont<NP>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, vec, out,
-                           plane_stride, ldo, rows_pad, k_pad);
+                           plane_stride, rows_pad, k_pad);
     } else {
         dim3 grid((unsigned)((rows_pad + 63) / 64), (unsigned)((k_pad + 63) / 64));
-        hipLaunchKernelGGL(split_prep_rcont<NP>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, plane_stride, ldo,
+        hipLaunchKernelGGL(split_prep_rcont<NP>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, plane_stride,
                            rows_pad, k_pad);
     }
     UGEMM_HIP_TRY(hipGetLastError());
@@ -403,8 +403,8 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     int rc = stream_alloc(&ws, bytes, s);
     if (rc) return rc;
     __bf16 *Ap = (__bf16 *)ws, *Bp = Ap + NP * a_plane;
-    rc = prep_operand<NP>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, Kp, (int)Mp, (int)Kp, s);
-    if (!rc) rc = prep_operand<NP>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, Kp, (int)Np, (int)Kp, s);
+    rc = prep_operand<NP>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, s);
+    if (!rc) rc = prep_operand<NP>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, s);
     if (!rc) {
         auto kern = bf16_nt_core<CF>;
         static bool attr = false;
@@ -416,7 +416,7 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
         }
         if (!rc) {
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
-                               (int)Kp, Ap, a_plane, Kp, Bp, b_plane, Kp, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n);
+                               (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = hip_fail(e, "bf16_nt_core launch");
         }
