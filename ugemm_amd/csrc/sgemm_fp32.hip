@@ -20,77 +20,6 @@ namespace {
 
 constexpr int LDS_PAD = 4;   // floats; keeps rows 16-byte aligned
 
-// Loads one operand tile (a C_EXT x S_EXT block: C_EXT elements along the
-// operand's contiguous dimension, S_EXT along its strided dimension) from
-// global memory into registers, VEC contiguous elements per chunk.
-template <int C_EXT, int S_EXT, int NT, int VEC>
-struct TileRegs {
-    static constexpr int CHUNKS_C = C_EXT / VEC;
-    static constexpr int NCHUNK = (C_EXT * S_EXT / VEC) / NT;
-    static_assert((C_EXT * S_EXT / VEC) % NT == 0, "tile must split evenly over threads");
-    float v[NCHUNK][VEC];
-
-    // base points at the tile origin; element (c, s) is base[s * ld + c].
-    // cmax / smax: valid extents from the origin (only used when CHECK).
-    template <bool CHECK>
-    __device__ __forceinline__ void load(const float *__restrict__ base, long ld, int t, int cmax, int smax) {
-#pragma unroll
-        for (int i = 0; i < NCHUNK; ++i) {
-            const int f = t + i * NT;
-            const int c = (f % CHUNKS_C) * VEC;
-            const int s = f / CHUNKS_C;
-            const float *p = base + (long)s * ld + c;
-            if (!CHECK) {
-                if constexpr (VEC == 4) {
-                    float4 x = *reinterpret_cast<const float4 *>(p);
-                    v[i][0] = x.x; v[i][1] = x.y; v[i][2] = x.z; v[i][3] = x.w;
-                } else {
-#pragma unroll
-                    for (int j = 0; j < VEC; ++j) v[i][j] = p[j];
-                }
-            } else {
-                const bool srow = s < smax;
-                if constexpr (VEC == 4) {
-                    if (srow && c + 4 <= cmax) {
-                        float4 x = *reinterpret_cast<const float4 *>(p);
-                        v[i][0] = x.x; v[i][1] = x.y; v[i][2] = x.z; v[i][3] = x.w;
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < VEC; ++j) v[i][j] = (srow && c + j < cmax) ? p[j] : 0.0f;
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < VEC; ++j) v[i][j] = (srow && c + j < cmax) ? p[j] : 0.0f;
-                }
-            }
-        }
-    }
-
-    // Writes the registers into an LDS array L[k][x] of row length LD.
-    // DIRECT: the contiguous dimension is x (the LDS fast index);
-    // otherwise it is k and the chunk is written down a column.
-    template <bool DIRECT, int LD>
-    __device__ __forceinline__ void store(float *L, int t) const {
-#pragma unroll
-        for (int i = 0; i < NCHUNK; ++i) {
-            const int f = t + i * NT;
-            const int c = (f % CHUNKS_C) * VEC;
-            const int s = f / CHUNKS_C;
-            if constexpr (DIRECT) {
-                if constexpr (VEC == 4) {
-                    *reinterpret_cast<float4 *>(&L[s * LD + c]) = make_float4(v[i][0], v[i][1], v[i][2], v[i][3]);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < VEC; ++j) L[s * LD + c + j] = v[i][j];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < VEC; ++j) L[(c + j) * LD + s] = v[i][j];
-            }
-        }
-    }
-};
-
 // Empty register constraints on the accumulators: the FMAs of one k-step
 // must complete before the next step's code, which bounds live ranges.
 template <int TM, int TN>
@@ -132,8 +61,8 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
     const int ty = (lane >> 3) + 8 * (w / WX);
 
     // A tile: TA -> stored K x M (contiguous m); else M x K (contiguous k).
-    using ARegs = TileRegs<TA ? BM : BK, TA ? BK : BM, NT, VEC>;
-    using BRegs = TileRegs<TB ? BK : BN, TB ? BN : BK, NT, VEC>;
+    using ARegs = TileRegs<float, TA ? BM : BK, TA ? BK : BM, NT, VEC>;
+    using BRegs = TileRegs<float, TB ? BK : BN, TB ? BN : BK, NT, VEC>;
     ARegs ra;
     BRegs rb;
 
