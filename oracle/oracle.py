@@ -217,3 +217,63 @@ def rel_errors(res, exact):
     nz = exact != 0
     rel = np.abs(res[nz] - exact[nz]) / np.abs(exact[nz])
     return (float(rel.max()), float(rel.mean())) if rel.size else (0.0, 0.0)
+
+
+# ---------------------------------------------------------------------------
+# FP8 (e4m3, OCP "E4M3FN": bias 7, no infinities, max 448, subnormal step
+# 2^-9) with one power-of-two scale per 128 x 128 block.  The reference has
+# no FP8 path; this restates the build's own quantisation rule so that the
+# FP8 mode can be checked element by element.  Pinned by the e4m3 definition
+# (tests/test_oracle.py: known encodings, rounding ties, subnormals).
+
+def e4m3_table():
+    """All finite non-negative e4m3 values in code order (code = 8*E + M)."""
+    vals = []
+    for e in range(16):
+        for m in range(8):
+            if e == 15 and m == 7:
+                continue  # NaN encoding
+            vals.append(m * 2.0 ** -9 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7))
+    return np.array(vals)
+
+
+_E4M3 = None
+
+
+def quantize_e4m3(x):
+    """Round-to-nearest-even onto e4m3 (|x| <= 448 assumed, larger saturates)."""
+    global _E4M3
+    if _E4M3 is None:
+        _E4M3 = e4m3_table()
+    v = _E4M3
+    x = np.asarray(x, np.float64)
+    a = np.minimum(np.abs(x), v[-1])
+    hi = np.clip(np.searchsorted(v, a, side="left"), 0, len(v) - 1)
+    lo = np.clip(hi - 1, 0, len(v) - 1)
+    d_lo, d_hi = a - v[lo], v[hi] - a
+    pick_hi = (d_hi < d_lo) | ((d_hi == d_lo) & (hi % 2 == 0))
+    q = np.where(pick_hi, v[hi], v[lo])
+    return np.sign(x) * q
+
+
+def block_exponent(amax):
+    """Smallest e with amax / 2^e <= 448 (0 for an all-zero block)."""
+    if not amax > 0:
+        return 0
+    m, e = np.frexp(amax / 448.0)
+    if m == 0.5:
+        e -= 1
+    return int(min(max(e, -127), 127))
+
+
+def fp8_block_dequant(X, block=128):
+    """X (rows x K, logical) -> the values the FP8 mode multiplies: each
+    block x block tile quantised with its power-of-two scale, then rescaled."""
+    X = np.asarray(X, np.float64)
+    out = np.empty_like(X)
+    for r0 in range(0, X.shape[0], block):
+        for k0 in range(0, X.shape[1], block):
+            t = X[r0:r0 + block, k0:k0 + block]
+            e = block_exponent(float(np.max(np.abs(t))) if t.size else 0.0)
+            out[r0:r0 + block, k0:k0 + block] = quantize_e4m3(t / 2.0 ** e) * 2.0 ** e
+    return out

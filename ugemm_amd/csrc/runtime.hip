@@ -198,7 +198,7 @@ static int choose_variant(const Gemm<float> &g, int mode) {
     if (v >= 0) return v;
     switch (mode) {
         case UGEMM_MODE_FP32: return fp32_pick_variant(g);
-        case UGEMM_MODE_FP8_BLOCK: return 0;
+        case UGEMM_MODE_FP8_BLOCK: return -1;   // launch_fp8 picks by grid size
         default: return mfma_pick_variant(g, mode);
     }
 }

@@ -1,7 +1,307 @@
-// FP8 block-scaled SGEMM (placeholder until the kernels land).
+// FP8-e4m3 SGEMM with one scale per 128 x 128 block (opt-in, reduced precision).
+//
+// Quantisation (per operand, HBM-bound preparation kernel): op(A) is cut into
+// 128 (rows) x 128 (k) blocks, op(B) into 128 (n) x 128 (k) blocks.  Each
+// block gets a power-of-two scale s = 2^e, e = ceil(log2(amax / 448)) (448 is
+// the largest finite e4m3 value; e = 0 for an all-zero block), and every
+// element becomes q = e4m3_rne(x / s) (subnormals kept, |x/s| <= 448 so no
+// saturation is needed).  Power-of-two scales are exact and are exactly what
+// the gfx950 block-scaled MFMA takes (E8M0 exponents), so
+//   C = alpha * sum_kb (sA[mb,kb] sB[nb,kb] * sum_{k in kb} qa qb) + beta * C
+// is computed by v_mfma_scale_f32_16x16x128_f8f6f4 with no extra VALU work,
+// accumulating in FP32.  Non-finite inputs give NaN (e4m3 has no infinity).
+//
+// Planes use the k-tile blocked layout of the split-BF16 path with 128-byte
+// rows (one K tile = one scale block): byte (r, k) of an operand with rows_pad
+// rows is at ((k / 128) * rows_pad + r) * 128 + ((k % 128) / 16 ^ (r & 7)) * 16
+// + k % 16.  The XOR makes the fragment reads (16 rows, one 16-byte chunk
+// each) conflict-free and lets the core copy a stage verbatim.
+// Fragment layout of the 16x16x128 fp8 MFMA: lane l holds row l%16 and k
+// 32*(l/16) .. +31 of A, the same k of column l%16 of B; the per-lane scale
+// operand is the E8M0 exponent of the block those elements belong to (one
+// value per wave and K tile here).  D holds rows 4*(l/16)+i, column l%16.
 #include "internal.h"
+#include "kernel_common.h"
+
 namespace ugemm {
-int fp8_variant_count() { return 0; }
-const char *fp8_variant_name(int) { return nullptr; }
-int launch_fp8(const Gemm<float> &, int, hipStream_t) { return fail(UGEMM_ERR_UNSUPPORTED, "fp8 not built yet"); }
+namespace {
+
+typedef int int8v __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int SB = 128;   // scale block edge = K tile
+
+__device__ __forceinline__ long fp8_blocked_offset(int r, int k, int rows_pad) {
+    return ((long)(k >> 7) * rows_pad + r) * 128 + ((((k >> 4) & 7) ^ (r & 7)) << 4);
+}
+
+// Power-of-two exponent for a block maximum (see the header comment).
+__device__ __forceinline__ int block_exponent(float amax) {
+    if (!(amax > 0.0f) || !__builtin_isfinite(amax)) return 0;
+    int e;
+    const float f = frexpf(amax / 448.0f, &e);   // amax/448 = f * 2^e, f in [0.5, 1)
+    if (f == 0.5f) e -= 1;                        // exact power of two: 2^(e-1)
+    return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+
+// Packs 4 floats (already divided by the scale) into 4 e4m3 bytes, RNE.
+__device__ __forceinline__ int pack4_e4m3(float a, float b, float c, float d) {
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+    return w;
+}
+
+// One workgroup per 128 x 128 block.  Source element (r, k): KCONT ? X[r*ld+k]
+// : X[k*ld+r]; zeros outside rows x K.  Writes the quantised block and its
+// exponent exps[(r / 128) * kblocks + k / 128].
+template <bool KCONT>
+__global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                      unsigned char *__restrict__ out, int rows_pad,
+                                                      int *__restrict__ exps, int kblocks) {
+    __shared__ float red[256];
+    __shared__ __attribute__((aligned(16))) unsigned char tile[128][128 + 16];
+    const int rb = blockIdx.x, kb = blockIdx.y;
+    const int r0 = rb * SB, k0 = kb * SB;
+    const int t = threadIdx.x;
+    // Each thread reads 64 elements: KCONT -> row t/2, k (t%2)*64..+63;
+    // else k t/2, rows (t%2)*64..+63 (coalesced along the contiguous index).
+    float v[64];
+    const int a = t >> 1, b0 = (t & 1) * 64;
+    float amax = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = b0 + q * 4;
+        const int r = KCONT ? r0 + a : r0 + c, k = KCONT ? k0 + c : k0 + a;
+        const float *p = KCONT ? X + (long)r * ld + k : X + (long)k * ld + r;
+        const bool full = KCONT ? (r < rows && k + 4 <= K) : (k < K && r + 4 <= rows);
+        if (full && vec) {
+            const float4 x = *reinterpret_cast<const float4 *>(p);
+            v[q * 4 + 0] = x.x; v[q * 4 + 1] = x.y; v[q * 4 + 2] = x.z; v[q * 4 + 3] = x.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = KCONT ? (r < rows && k + j < K) : (k < K && r + j < rows);
+                v[q * 4 + j] = ok ? p[j] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) amax = fmaxf(amax, fabsf(v[q * 4 + j])) ;
+    }
+    // NaN/Inf make the block non-finite: keep it visible in amax.
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        if (!__builtin_isfinite(v[j])) amax = __builtin_inff();
+    red[t] = amax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmaxf(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const float bmax = red[0];
+    const int e = block_exponent(bmax);
+    const float inv = ldexpf(1.0f, -e);
+    if (t == 0) exps[(long)rb * kblocks + kb] = e;
+    // Quantise into the LDS tile [r][k] (bytes), then write rows out.
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int w = pack4_e4m3(v[q * 4] * inv, v[q * 4 + 1] * inv, v[q * 4 + 2] * inv, v[q * 4 + 3] * inv);
+        if (KCONT) {
+            *reinterpret_cast<int *>(&tile[a][b0 + q * 4]) = w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tile[b0 + q * 4 + j][a] = (unsigned char)((unsigned)w >> (8 * j));
+        }
+    }
+    __syncthreads();
+    // 128 rows x 8 chunks of 16 bytes = 1024 chunks, 4 per thread.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int f = t + i * 256, r = f >> 3, c = f & 7;
+        const uint4 x = *reinterpret_cast<const uint4 *>(&tile[r][c * 16]);
+        *reinterpret_cast<uint4 *>(out + fp8_blocked_offset(r0 + r, k0 + c * 16, rows_pad)) = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Core: 256 x 256 tiles, 4 waves (2 x 2) of 128 x 128, K tile 128.
+struct F8Cfg {
+    static constexpr int BM = 256, BN = 256, BK = 128, NT = 256, WTM = 128, WTN = 128, FM = 8, FN = 8;
+    static constexpr int STAGE = (BM + BN) * BK;                  // bytes
+    static constexpr int LDS_BYTES = 2 * STAGE;
+    static constexpr int PER_T = STAGE / 16 / NT;                 // 16-byte chunks per thread
+};
+struct F8CfgSmall {
+    static constexpr int BM = 128, BN = 128, BK = 128, NT = 256, WTM = 64, WTN = 64, FM = 4, FN = 4;
+    static constexpr int STAGE = (BM + BN) * BK;
+    static constexpr int LDS_BYTES = 2 * STAGE;
+    static constexpr int PER_T = STAGE / 16 / NT;
+};
+
+template <class CF>
+__global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, const unsigned char *__restrict__ Aq,
+                                                          int Mp, const int *__restrict__ ea,
+                                                          const unsigned char *__restrict__ Bq, int Np,
+                                                          const int *__restrict__ eb, float alpha, float beta,
+                                                          float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, FM = CF::FM, FN = CF::FN, PER_T = CF::PER_T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem8[];
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int kblocks = Kp / BK;
+
+    // Stage kt: A rows m0.. (BM * 128 contiguous bytes), then B rows n0...
+    constexpr int A_CH = BM * BK / 16 / CF::NT;
+    const unsigned char *a_src = Aq + (long)m0 * BK + t * 16;
+    const unsigned char *b_src = Bq + (long)n0 * BK + t * 16;
+    uint4 regs[PER_T];
+    auto load_stage = [&](int kt) {
+        const long ak = (long)kt * Mp * BK, bk = (long)kt * Np * BK;
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) {
+            if (i < A_CH) regs[i] = *reinterpret_cast<const uint4 *>(a_src + ak + i * CF::NT * 16);
+            else regs[i] = *reinterpret_cast<const uint4 *>(b_src + bk + (i - A_CH) * CF::NT * 16);
+        }
+    };
+    auto store_stage = [&](int buf) {
+        unsigned char *base = smem8 + buf * CF::STAGE + t * 16;
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<uint4 *>(base + i * CF::NT * 16) = regs[i];
+    };
+
+    floatx4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    load_stage(0);
+    store_stage(0);
+    __syncthreads();
+    const int frow = lane & 15, fq = lane >> 4;
+    // Scale blocks of this wave: rows (m0 + wm*WTM) / 128, columns likewise.
+    const int a_blk = (m0 + wm * CF::WTM) / SB, b_blk = (n0 + wn * CF::WTN) / SB;
+    for (int kt = 0; kt < kblocks; ++kt) {
+        const int buf = kt & 1;
+        const bool more = kt + 1 < kblocks;
+        if (more) load_stage(kt + 1);
+        const int sa = ea[(long)a_blk * kblocks + kt] + 127, sb = eb[(long)b_blk * kblocks + kt] + 127;
+        const unsigned char *as = smem8 + buf * CF::STAGE;
+        const unsigned char *bs = as + BM * BK;
+        int8v fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int row = wm * CF::WTM + i * 16 + frow;
+            const uint4 lo = *reinterpret_cast<const uint4 *>(as + row * BK + (((2 * fq) ^ (row & 7)) << 4));
+            const uint4 hi = *reinterpret_cast<const uint4 *>(as + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
+            fa[i] = int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int row = wn * CF::WTN + j * 16 + frow;
+            const uint4 lo = *reinterpret_cast<const uint4 *>(bs + row * BK + (((2 * fq) ^ (row & 7)) << 4));
+            const uint4 hi = *reinterpret_cast<const uint4 *>(bs + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
+            fb[j] = int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sa, 0,
+                                                                             sb);
+        if (more) store_stage(buf ^ 1);
+        __syncthreads();
+    }
+
+    const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4 + r;
+                if (interior || (m < M && n < N)) {
+                    float *cp = C + (long)m * ldc + n;
+                    *cp = (beta == 0.0f) ? alpha * acc[i][j][r] : fmaf(alpha, acc[i][j][r], beta * *cp);
+                }
+            }
+        }
+    }
+}
+
+long round_up(long x, long a) { return (x + a - 1) / a * a; }
+
+int prep(const float *X, long ld, bool kcont, int rows, int K, unsigned char *out, int rows_pad, int k_pad, int *exps,
+         hipStream_t s) {
+    const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
+    dim3 grid((unsigned)(rows_pad / SB), (unsigned)(k_pad / SB));
+    if (kcont)
+        hipLaunchKernelGGL(fp8_block_prep<true>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, rows_pad, exps,
+                           k_pad / SB);
+    else
+        hipLaunchKernelGGL(fp8_block_prep<false>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, rows_pad, exps,
+                           k_pad / SB);
+    UGEMM_HIP_TRY(hipGetLastError());
+    return UGEMM_OK;
+}
+
+template <class CF>
+int run(const Gemm<float> &g, hipStream_t s) {
+    const long Mp = round_up(g.M, CF::BM), Np = round_up(g.N, CF::BN), Kp = round_up(g.K, SB);
+    const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
+    if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
+    const long kb = Kp / SB;
+    const size_t a_bytes = (size_t)Mp * Kp, b_bytes = (size_t)Np * Kp;
+    const size_t e_bytes = (size_t)((Mp + Np) / SB) * kb * sizeof(int);
+    void *ws = nullptr;
+    int rc = stream_alloc(&ws, a_bytes + b_bytes + e_bytes, s);
+    if (rc) return rc;
+    unsigned char *Aq = (unsigned char *)ws, *Bq = Aq + a_bytes;
+    int *ea = (int *)(Bq + b_bytes), *eb = ea + (Mp / SB) * kb;
+    rc = prep(g.A, g.lda, !g.ta, g.M, g.K, Aq, (int)Mp, (int)Kp, ea, s);
+    if (!rc) rc = prep(g.B, g.ldb, g.tb, g.N, g.K, Bq, (int)Np, (int)Kp, eb, s);
+    if (!rc) {
+        auto kern = fp8_scaled_core<CF>;
+        static bool attr = false;
+        if (!attr) {
+            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               CF::LDS_BYTES);
+            if (e != hipSuccess) rc = hip_fail(e, "hipFuncSetAttribute");
+            attr = true;
+        }
+        if (!rc) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
+                               (int)Kp, Aq, (int)Mp, ea, Bq, (int)Np, eb, g.alpha, g.beta, g.C, g.ldc, tiles_m,
+                               tiles_n);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = hip_fail(e, "fp8_scaled_core launch");
+        }
+    }
+    int rc2 = stream_free(ws, s);
+    return rc ? rc : rc2;
+}
+
+const char *kNames[] = {"fp8_e4m3_blk128_256x256x128", "fp8_e4m3_blk128_128x128x128"};
+
+}  // namespace
+
+int fp8_variant_count() { return 2; }
+const char *fp8_variant_name(int v) { return (v >= 0 && v < 2) ? kNames[v] : nullptr; }
+
+int launch_fp8(const Gemm<float> &g, int variant, hipStream_t s) {
+    if (variant < 0) {
+        const long t256 = (long)((g.M + 255) / 256) * ((g.N + 255) / 256);
+        variant = t256 >= device_cu_count() ? 0 : 1;
+    }
+    switch (variant) {
+        case 0: return run<F8Cfg>(g, s);
+        case 1: return run<F8CfgSmall>(g, s);
+        default: return fail(UGEMM_ERR_ARG, "fp8: no variant %d", variant);
+    }
+}
+
 }  // namespace ugemm
