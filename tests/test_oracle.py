@@ -1,0 +1,260 @@
+"""Pins the CPU oracle (oracle/oracle.c, oracle/oracle.py) -- CPU only.
+
+Checks against
+  * the known-answer product of sgemm_test.c:186-200;
+  * the reference's own outputs (tests/golden/ref_gemm.npz, written by
+    tests/golden/make_golden.py from the reference compiled from its sources);
+  * exact small-integer products and numpy's FP64 matmul (an independent
+    library routine);
+  * planted mistakes (transposed operand, dropped term, wrong sign, wrong
+    beta): the acceptance check must reject each of them;
+  * the e4m3 definition (known encodings, ties, subnormals) for the FP8 emulation.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+COMBOS = [(m, a, b) for m in "RC" for a in "NT" for b in "NT"]
+
+
+def make(major, ta, tb, M, N, K, dtype=np.float32, pad=0, seed=1, gen=orc.uniform):
+    row = major == "R"
+    la = (M if ta == "T" else K) if row else (K if ta == "T" else M)
+    lb = (K if tb == "T" else N) if row else (N if tb == "T" else K)
+    lc = N if row else M
+    lda, ldb, ldc = la + pad, lb + pad, lc + pad
+    g = orc.rng(seed)
+    A = gen(g, orc.stored_shape(major, ta, M, K, lda), dtype=dtype)
+    B = gen(g, orc.stored_shape(major, tb, K, N, ldb), dtype=dtype)
+    C = gen(g, (M, ldc) if row else (N, ldc), dtype=dtype)
+    return A, lda, B, ldb, C, ldc
+
+
+# ---------------------------------------------------------------------------
+def test_known_answer_all_oracle_paths():
+    A = np.arange(1, 7, dtype=np.float32)
+    B = np.arange(1, 7, dtype=np.float32)
+    want = [9, 12, 15, 19, 26, 33, 29, 40, 51]
+    for fn in (orc.sgemm_naive, orc.sgemm_packed):
+        C = np.zeros(9, np.float32)
+        fn("R", "N", "N", 3, 3, 2, 1.0, A, 2, B, 3, 0.0, C, 3)
+        assert C.tolist() == want
+    out, scale = orc.reference_f64("R", "N", "N", 3, 3, 2, 1.0, A, 2, B, 3, 0.0, None, 3)
+    assert out.ravel().tolist() == want
+
+
+@pytest.mark.parametrize("major,ta,tb", COMBOS)
+def test_naive_and_packed_match_numpy_fp64(major, ta, tb):
+    M, N, K = 23, 17, 41
+    for dtype in (np.float32, np.float64):
+        A, lda, B, ldb, C0, ldc = make(major, ta, tb, M, N, K, dtype=dtype, pad=3, seed=7)
+        opA = orc.op_matrix(major, ta, A, M, K).astype(np.float64)
+        opB = orc.op_matrix(major, tb, B, K, N).astype(np.float64)
+        want = 1.5 * (opA @ opB) + 0.5 * orc.c_matrix(major, C0, M, N).astype(np.float64)
+        scale = 1.5 * (np.abs(opA) @ np.abs(opB)) + 0.5 * np.abs(orc.c_matrix(major, C0, M, N))
+        eps = orc.EPS32 if dtype == np.float32 else orc.EPS64
+        for fn in ((orc.sgemm_naive, orc.sgemm_packed) if dtype == np.float32 else (orc.dgemm_naive, orc.dgemm_packed)):
+            C = C0.copy()
+            fn(major, ta, tb, M, N, K, 1.5, A, lda, B, ldb, 0.5, C, ldc)
+            ok, worst = orc.check_bound(orc.c_matrix(major, C, M, N), want, scale, orc.tolerance(K, eps=eps))
+            assert ok, (fn.__name__, worst)
+            row_len = N if major == "R" else M
+            assert np.array_equal(C[:, row_len:], C0[:, row_len:]), "padding written"
+        exact, sc = orc.reference_f64(major, ta, tb, M, N, K, 1.5, A, lda, B, ldb, 0.5, C0, ldc)
+        assert np.allclose(exact, want, rtol=1e-12, atol=1e-12 * np.abs(scale).max())
+        assert np.allclose(sc, scale, rtol=1e-12)
+
+
+def test_reference_f64_is_exact_on_integers_and_row_subsets():
+    M, N, K = 40, 30, 500
+    g = orc.rng(2)
+    A = g.integers(-1000, 1001, size=(M, K)).astype(np.float32)
+    B = g.integers(-1000, 1001, size=(K, N)).astype(np.float32)
+    exact = A.astype(np.int64) @ B.astype(np.int64)
+    out, _ = orc.reference_f64("R", "N", "N", M, N, K, 1.0, A, K, B, N, 0.0, None, N)
+    assert np.array_equal(out, exact.astype(np.float64))
+    rows = np.array([3, 17, 39])
+    sub, _ = orc.reference_f64("R", "N", "N", M, N, K, 1.0, A, K, B, N, 0.0, None, N, rows=rows)
+    assert np.array_equal(sub, out[rows])
+    # double-double: products of 2^30-range integers summed exactly
+    Ad = g.integers(-(1 << 30), 1 << 30, size=(8, 64)).astype(np.float64)
+    Bd = g.integers(-(1 << 30), 1 << 30, size=(64, 8)).astype(np.float64)
+    outd, _ = orc.reference_f64("R", "N", "N", 8, 8, 64, 1.0, Ad, 64, Bd, 8, 0.0, None, 8)
+    exact_d = [[sum(int(Ad[i, k]) * int(Bd[k, j]) for k in range(64)) for j in range(8)] for i in range(8)]
+    assert np.array_equal(outd, np.array(exact_d, dtype=np.float64))
+
+
+def test_planted_mistakes_are_rejected():
+    # The acceptance check must fail for plausible kernel mistakes.
+    M, N, K = 48, 40, 64
+    A, lda, B, ldb, C0, ldc = make("R", "N", "N", M, N, K, seed=3)
+    exact, scale = orc.reference_f64("R", "N", "N", M, N, K, 1.5, A, lda, B, ldb, 0.5, C0, ldc)
+    tol = orc.tolerance(K)
+    good = C0.copy()
+    orc.sgemm_naive("R", "N", "N", M, N, K, 1.5, A, lda, B, ldb, 0.5, good, ldc)
+    assert orc.check_bound(good, exact, scale, tol)[0]
+    A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
+    mistakes = {
+        "B transposed": 1.5 * (A64[:, :N] @ B64[:N, :N].T[:, :N]) if K >= N else None,
+        "dropped last k": 1.5 * (A64[:, :-1] @ B64[:-1]) + 0.5 * C64,
+        "wrong sign of beta term": 1.5 * (A64 @ B64) - 0.5 * C64,
+        "beta ignored": 1.5 * (A64 @ B64),
+        "alpha applied twice": 2.25 * (A64 @ B64) + 0.5 * C64,
+        "k index shifted": 1.5 * (A64[:, 1:] @ B64[:-1]) + 0.5 * C64,
+    }
+    for name, bad in mistakes.items():
+        if bad is None:
+            continue
+        assert not orc.check_bound(bad, exact, scale, tol)[0], name
+
+
+def test_cmp_results_statistic():
+    ref = np.array([[1.0, 2.0], [3.0, 4.0]], np.float32)
+    res = ref.copy()
+    res[1, 0] += 1e-3
+    std_err, std_ref, max_err, (i, j), verdict = orc.cmp_results(2, 2, ref, res, 2)
+    assert (i, j) == (1, 0) and abs(max_err - 1e-3) < 1e-6
+    assert abs(std_ref - np.sqrt(1.25)) < 1e-12
+    assert verdict == "FAIL !!!"
+
+
+def test_generators():
+    g = orc.rng(5)
+    u = orc.uniform(g, 10000)
+    k = (u.astype(np.float64) + 1.0) * 2.0 ** 23
+    assert np.all(k == np.round(k)) and u.min() >= -1.0 and u.max() < 1.0
+    w = orc.wide_range(orc.rng(6), 10000)
+    e = np.floor(np.log2(np.abs(w.astype(np.float64))))
+    assert e.min() == -12 and e.max() == 12
+
+
+# ---------------------------------------------------------------------------
+# Golden vectors from the compiled reference.
+
+def load_golden():
+    meta = json.load(open(os.path.join(GOLDEN, "ref_gemm.json")))
+    data = np.load(os.path.join(GOLDEN, "ref_gemm.npz"))
+    return meta["cases"], data
+
+
+def golden_inputs(case, data):
+    if case["inputs"] is not None:
+        p = case["inputs"]
+        return data[p + "_A"], data[p + "_B"], data[p + "_C0"]
+    dtype = np.dtype(case["dtype"])
+    g = orc.rng(case["seed"])
+    sa = orc.stored_shape(case["major"], case["ta"], case["M"], case["K"], case["lda"])
+    sb = orc.stored_shape(case["major"], case["tb"], case["K"], case["N"], case["ldb"])
+    sc = (case["M"], case["ldc"]) if case["major"] == "R" else (case["N"], case["ldc"])
+    A, B, C0 = orc.uniform(g, sa, dtype=dtype), orc.uniform(g, sb, dtype=dtype), orc.uniform(g, sc, dtype=dtype)
+    if hashlib.sha256(A.tobytes() + B.tobytes() + C0.tobytes()).hexdigest() != case["inputs_sha256"]:
+        pytest.skip("regenerated inputs differ from the fixture's (numpy stream changed)")
+    return A, B, C0
+
+
+def reference_divergence(case):
+    """Why the reference's result deliberately differs from BLAS semantics
+    (which the oracle and the GPU path follow), or None."""
+    fn, major = case["fn"], case["major"]
+    blocked = ("sgemm_c", "_dgemm_c", "ref_sgemm_sse", "dgemm_avx", "sgemm_avx")
+    scales_only = (case["alpha"] == 0.0 or case["K"] == 0) and case["beta"] != 1.0
+    if scales_only:
+        if fn in blocked and major == "R" and case["M"] != case["N"]:
+            return "alpha == 0 indexes C column-major (gemm_cpu.h:308-323)"
+        return None
+    if fn == "sgemm_avx" and (case["ta"] != "N" or case["tb"] != "N"):
+        return "sgemm_avx implements R/N/N only (sgemm_avx256.h:439-453)"
+    return None
+
+
+def test_golden_fixture_integrity():
+    meta = json.load(open(os.path.join(GOLDEN, "ref_gemm.json")))
+    h = hashlib.sha256(open(os.path.join(GOLDEN, "ref_gemm.npz"), "rb").read()).hexdigest()
+    assert h == meta["sha256"]
+    fns = {c["fn"] for c in meta["cases"]}
+    assert fns == {"sgemm_cpu", "sgemm_c", "sgemm_avx", "ref_sgemm_sse", "dgemm_cpu", "_dgemm_c", "dgemm_avx"}
+
+
+def test_golden_known_answer():
+    cases, data = load_golden()
+    kat = [i for i, c in enumerate(cases) if c["M"] == 3 and c["N"] == 3 and c["K"] == 2 and c["inputs"] is not None
+           and np.array_equal(data[c["inputs"] + "_A"].ravel(), np.arange(1, 7))]
+    assert len(kat) >= 4
+    for i in kat:
+        assert data[f"C{i}"].ravel().tolist() == [9, 12, 15, 19, 26, 33, 29, 40, 51]
+
+
+def test_oracle_matches_reference_golden_vectors():
+    cases, data = load_golden()
+    checked = diverged = 0
+    for i, c in enumerate(cases):
+        A, B, C0 = golden_inputs(c, data)
+        ref_out = data[f"C{i}"]
+        why = reference_divergence(c)
+        eps = orc.EPS32 if c["dtype"] == "float32" else orc.EPS64
+        args = (c["major"], c["ta"], c["tb"], c["M"], c["N"], c["K"])
+        exact, scale = orc.reference_f64(*args, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C0, c["ldc"])
+        got_ref = orc.c_matrix(c["major"], ref_out, c["M"], c["N"])
+        ok, worst = orc.check_bound(got_ref, exact, scale, orc.tolerance(c["K"], eps=eps))
+        if why is None:
+            assert ok, (c, worst)
+            assert not c["writes_outside_C"], c
+            checked += 1
+        else:
+            if "R/N/N only" in why:
+                assert np.array_equal(ref_out, C0), c       # C left untouched
+            diverged += 1
+        # The naive restatement follows sgemm_cpu / dgemm_cpu exactly in
+        # semantics (same index formulas, C read even when beta == 0).
+        if c["fn"] in ("sgemm_cpu", "dgemm_cpu"):
+            mine = C0.copy()
+            (orc.sgemm_naive if c["dtype"] == "float32" else orc.dgemm_naive)(
+                *args, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], mine, c["ldc"])
+            ok2, w2 = orc.check_bound(orc.c_matrix(c["major"], mine, c["M"], c["N"]), got_ref.astype(np.float64),
+                                      scale, orc.tolerance(c["K"], eps=eps))
+            assert ok2, (c, w2)
+    assert checked > 1000 and diverged > 0
+
+
+def test_reference_alpha_zero_bug_is_real():
+    # The documented divergence: the reference's blocked paths write outside
+    # the row-major C for alpha == 0 and M != N (captured with a canary).
+    cases, _ = load_golden()
+    bad = [c for c in cases if c["writes_outside_C"]]
+    assert bad and all(c["alpha"] == 0.0 and c["major"] == "R" and c["M"] < c["N"] for c in bad)
+
+
+# ---------------------------------------------------------------------------
+# e4m3 (FP8) emulation.
+
+def test_e4m3_table_and_rounding():
+    v = orc.e4m3_table()
+    assert len(v) == 127 and v[-1] == 448.0 and v[1] == 2.0 ** -9 and v[8] == 2.0 ** -6
+    q = orc.quantize_e4m3
+    assert q(448.0) == 448.0 and q(460.0) == 448.0
+    assert q(17.0) == 16.0                        # tie between 16 and 18 -> even mantissa
+    assert q(19.0) == 20.0                        # tie between 18 and 20 -> 20 (mantissa 2 vs 1)
+    assert q(1.0625) == 1.0 and q(1.1875) == 1.25
+    assert q(2.0 ** -10) == 0.0                   # half the smallest subnormal: tie -> 0
+    assert q(3 * 2.0 ** -10) == 2.0 ** -8         # tie between 1 and 2 subnormal steps -> 2
+    assert q(-0.3) == -0.3125
+    assert orc.block_exponent(448.0) == 0 and orc.block_exponent(449.0) == 1
+    assert orc.block_exponent(1.0) == -8 and orc.block_exponent(0.0) == 0
+
+
+def test_fp8_block_dequant_uses_one_scale_per_block():
+    X = np.zeros((130, 256))
+    X[0, 0] = 448.0 * 4          # block (0, 0): scale 4
+    X[0, 1] = 5.0                # quantised with the block scale: 5/4 -> 1.25 -> 5.0
+    X[129, 200] = 1.0            # block (1, 1): scale 2^-8
+    X[129, 201] = 0.3
+    D = orc.fp8_block_dequant(X)
+    assert D[0, 0] == 1792.0 and D[0, 1] == 5.0
+    assert D[129, 200] == 1.0 and D[129, 201] == orc.quantize_e4m3(0.3 * 256) / 256
