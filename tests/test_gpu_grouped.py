@@ -1,0 +1,138 @@
+"""Grouped SGEMM (one persistent launch over many uneven problems, optionally
+replayed as a hipGraph): every problem's C must meet the FP32 bound against
+the oracle, replays must be bitwise identical, tuning must keep results."""
+import numpy as np
+import pytest
+
+import ugemm_amd as ug
+from ugemm_amd import hip
+from oracle import oracle as orc
+from tests.helpers import Case
+
+pytestmark = pytest.mark.gpu
+
+
+class DevCase:
+    def __init__(self, case):
+        self.case = case
+        self.dA = hip.DeviceArray.from_host(case.A)
+        self.dB = hip.DeviceArray.from_host(case.B)
+        self.dC = hip.DeviceArray.from_host(case.C0)
+
+    def problem(self):
+        c = self.case
+        return ug.Problem(c.ta.encode(), c.tb.encode(), c.M, c.N, c.K, c.alpha, c.beta, self.dA.ptr, c.lda,
+                          self.dB.ptr, c.ldb, self.dC.ptr, c.ldc)
+
+    def reset(self):
+        self.dC.upload(self.case.C0)
+
+    def check(self):
+        C = self.dC.download()
+        return self.case.judge(C)
+
+
+def uneven_cases(n, seed, max_cin=64, sizes=(7, 14, 28)):
+    # im2col-shaped problems (the conv -> GEMM mapping of sgemm_ocl1.h:271-300):
+    # M = Cout, N = H*W, K = Cin*3*3, mixed layouts, alpha/beta and ld padding.
+    g = orc.rng(seed)
+    out = []
+    for i in range(n):
+        M = 32 * int(g.integers(1, 9))
+        hw = int(g.choice(sizes))
+        N = hw * hw
+        K = 9 * 16 * int(g.integers(1, max_cin // 16 + 1))
+        ta, tb = ("N", "T")[i % 2], ("N", "T")[(i // 2) % 2]
+        alpha, beta = ((1.0, 0.0), (1.5, 0.5), (-1.0, 1.0))[i % 3]
+        out.append(Case("R", ta, tb, M, N, K, alpha, beta, seed=seed * 1000 + i, pad=(i % 3)))
+    # edge problems: tiny, K = 0, alpha = 0
+    out.append(Case("R", "N", "N", 1, 1, 1, 1.0, 0.0, seed=seed + 7))
+    out.append(Case("R", "N", "N", 33, 17, 0, 1.0, 0.5, seed=seed + 8))
+    out.append(Case("R", "T", "N", 40, 50, 60, 0.0, 0.25, seed=seed + 9))
+    return out
+
+
+@pytest.mark.parametrize("capture", [False, True])
+def test_grouped_matches_oracle(capture):
+    cases = [DevCase(c) for c in uneven_cases(40, 5)]
+    plan = ug.GroupedPlan([d.problem() for d in cases], capture=capture)
+    s = hip.Stream()
+    plan.run(s)
+    s.synchronize()
+    for d in cases:
+        ok, worst = d.check()
+        assert ok, (d.case.M, d.case.N, d.case.K, d.case.ta, d.case.tb, worst)
+    plan.destroy()
+    s.destroy()
+
+
+def test_graph_replays_are_identical():
+    cases = [DevCase(c) for c in uneven_cases(24, 9)]
+    for d in cases:
+        d.case.beta = 0.0
+    plan = ug.GroupedPlan([d.problem() for d in cases], capture=True)
+    s = hip.Stream()
+    first = None
+    for r in range(3):
+        plan.run(s)
+        s.synchronize()
+        now = [d.dC.download() for d in cases]
+        if first is None:
+            first = now
+        else:
+            for a, b in zip(first, now):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        # eager work between replays
+        ug.sgemm_dev("R", "N", "N", 64, 64, 64, 1.0, cases[0].dA, cases[0].case.lda, cases[0].dB,
+                     cases[0].case.ldb, 0.0, hip.DeviceArray((64, 64), np.float32), 64, stream=s) \
+            if cases[0].case.ta == "N" and cases[0].case.lda >= 64 and cases[0].case.M >= 64 else None
+        s.synchronize()
+    for d in cases:
+        ok, worst = d.check()
+        assert ok, worst
+    plan.destroy()
+    s.destroy()
+
+
+def test_grouped_tuning_keeps_results():
+    cases = [DevCase(c) for c in uneven_cases(16, 11)]
+    plan = ug.GroupedPlan([d.problem() for d in cases], capture=True)
+    v = plan.tune()
+    assert 0 <= v < 4
+    for d in cases:
+        d.reset()
+    s = hip.Stream()
+    plan.run(s)
+    s.synchronize()
+    for d in cases:
+        assert d.check()[0]
+    plan.destroy()
+    s.destroy()
+
+
+@pytest.mark.parametrize("variant", range(4))
+def test_grouped_every_variant(variant):
+    ug.force_variant(ug.MODE_FP32, variant)
+    try:
+        cases = [DevCase(c) for c in uneven_cases(12, 13 + variant)]
+        ug.load().ugemm_hip_sgemm_grouped(len(cases), (ug.Problem * len(cases))(*[d.problem() for d in cases]),
+                                          ug.MODE_FP32, None)
+        hip.synchronize()
+        for d in cases:
+            assert d.check()[0]
+    finally:
+        ug.force_variant(ug.MODE_FP32, -1)
+
+
+def test_grouped_argument_errors():
+    L = ug.load()
+    bad = ug.Problem(b"X", b"N", 4, 4, 4, 1.0, 0.0, 1, 4, 1, 4, 1, 4)
+    arr = (ug.Problem * 1)(bad)
+    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP32, None) == ug.ERR_ARG
+    bad = ug.Problem(b"N", b"N", 4, 4, 4, 1.0, 0.0, None, 4, None, 4, None, 4)
+    arr = (ug.Problem * 1)(bad)
+    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP32, None) == ug.ERR_ARG
+    ok = ug.Problem(b"N", b"N", 4, 4, 4, 1.0, 0.0, 16, 4, 16, 4, 16, 4)
+    arr = (ug.Problem * 1)(ok)
+    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_BF16X3, None) == ug.ERR_UNSUPPORTED
+    assert L.ugemm_hip_sgemm_grouped(0, None, ug.MODE_FP32, None) == ug.OK

@@ -58,7 +58,9 @@ int launch_dgemm(const Gemm<double> &g, int variant, hipStream_t s);
 int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s);
 int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s);
 
-// ---- tuning (tune.cpp) ----
+// ---- tuning (tune.hip) ----
+// Tuning-table key of grouped plans (variants = the 4 FP32 tile shapes).
+constexpr int TUNE_KEY_GROUPED = 16;
 int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K);   // -1 if none
 int forced_variant(int mode);                                         // -1 if none
 void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms);

@@ -2,7 +2,9 @@
 // remembers the fastest per shape bucket; the table persists as a small text
 // file (safe to delete: lookups then fall back to the built-in heuristics).
 //
-// Bucket key: (mode, transa, transb, ceil(log2 M), ceil(log2 N), ceil(log2 K)).
+// Bucket key: (mode, transa, transb, ceil(log2 M), ceil(log2 N), ceil(log2 K));
+// grouped plans use key TUNE_KEY_GROUPED with (count, MFLOP, 0) in place of
+// (M, N, K).
 // File format (version 1):
 //   ugemm-hip-tune 1 <gcnArch> <CUs>
 //   <mode> <ta> <tb> <lgM> <lgN> <lgK> <variant> <ms>
@@ -197,7 +199,9 @@ int ugemm_hip_tune_load(const char *path) {
         int m, a, b, lm, ln, lk, v;
         float ms;
         while (fscanf(f, "%d %d %d %d %d %d %d %f", &m, &a, &b, &lm, &ln, &lk, &v, &ms) == 8) {
-            if (m < -1 || m >= UGEMM_MODE_COUNT || v < 0 || v >= ugemm_hip_variant_count(m)) continue;
+            const int nv = (m == TUNE_KEY_GROUPED) ? ugemm_hip_variant_count(UGEMM_MODE_FP32)
+                                                   : ((m < -1 || m >= UGEMM_MODE_COUNT) ? 0 : ugemm_hip_variant_count(m));
+            if (v < 0 || v >= nv) continue;
             std::lock_guard<std::mutex> lk2(g_mu);
             g_table[Key(m, a, b, lm, ln, lk)] = Entry{v, ms};
             ++loaded;
