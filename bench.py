@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the SGEMM hot path on one MI355X.
+"""Benchmark of the SGEMM hot path on one MI355X (gfx950).
 
-Workload: square SGEMM, M = N = K = 8192, row-major NN, alpha = 1.5,
-beta = 0.5, A, B, C ~ U[-1, 1] (seeded), inputs resident in HBM.  A "step" is
-one GEMM call of the headline form (split-BF16 on MFMA); the plain FP32 form
-and the other forms are timed the same way and reported beside it.
+Headline workload: SGEMM M = N = K = 8192, row-major NN, alpha = 1.5,
+beta = 0.5, A, B, C ~ U[-1, 1) (24-bit, seeded), inputs resident in HBM.
+A "step" is one call of the headline form, the split-BF16 x3 SGEMM on MFMA;
+the plain FP32 kernel and the other forms (BF16x6, BF16x1, FP8 block-scaled)
+are timed the same way beside it.  Timing: W untimed warm-up calls, then K
+calls bracketed by device synchronisation and HIP events on the launch stream.
+The dominant kernel's own average duration (the MFMA core, without the
+operand-preparation kernels) comes from the library's event hook
+(ugemm_hip_timing_*), recorded on the same stream.
 
-Prints ONE JSON line (rank 0).  Timing: W untimed warm-up calls, then K calls
-bracketed by device synchronisation, measured with HIP events recorded on the
-stream the kernels run on.  Errors: max / mean relative error against an FP64
-evaluation of 64 sampled rows.  cpu_baseline: the reference's own AVX SGEMM
-(oracle/_ref, compiled from its sources) on a bounded slice of the same
-workload, one thread, as the reference runs.
+Also reported (one JSON line in total):
+  * errors of every form vs an FP64 evaluation of 64 sampled rows;
+  * tall-skinny / short-wide SGEMM and the 4096^3 DGEMM (normal(0,1) inputs,
+    DGEMM bit-compared across two runs);
+  * 512 uneven im2col-shaped problems in one grouped persistent launch
+    (hipGraph replay) vs a loop of plain-kernel launches;
+  * the precision ladder at 8192^3 with entries sign * 2^e * (1+u), e in [-12, 12];
+  * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
+    into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread.
 """
 import argparse
 import json
@@ -28,73 +36,248 @@ sys.path.insert(0, ROOT)
 import ugemm_amd as ug  # noqa: E402
 from ugemm_amd import hip  # noqa: E402
 
-PEAK = {"fp32_vector": 157.3, "bf16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}  # TFLOP/s, MI355X
-MODE_PEAK = {"fp32": ("fp32_vector", 1), "bf16x3": ("bf16_mfma", 3), "bf16x6": ("bf16_mfma", 6),
+# MI355X published dense peaks (TFLOP/s) and HBM bandwidth.
+PEAK = {"fp32_vector": 157.3, "bf16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}
+HBM_GBPS = 8000.0
+# Form -> (peak used for roofline.frac, MFMA products per useful product)
+FORM_PEAK = {"fp32": ("fp32_vector", 1), "bf16x3": ("bf16_mfma", 3), "bf16x6": ("bf16_mfma", 6),
              "bf16x1": ("bf16_mfma", 1), "fp8": ("fp8_mfma", 1)}
+CORE_KERNEL = {"fp32": "sgemm_fp32_kernel", "bf16x3": "bf16_nt_core", "bf16x6": "bf16_nt_core",
+               "bf16x1": "bf16_nt_core", "fp8": "fp8_scaled_core"}
 
 
-def gen_inputs(n, seed):
-    g = np.random.Generator(np.random.PCG64(seed))
-
-    def u(shape):
-        return (g.integers(0, 1 << 24, size=shape, dtype=np.int64).astype(np.float32) * np.float32(2.0 ** -23)
-                - np.float32(1.0))
-    return u((n, n)), u((n, n)), u((n, n))
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
-def time_mode(mode_id, n, dA, dB, dC, alpha, beta, steps, warmup, stream):
+def uniform(g, shape):
+    k = g.integers(0, 1 << 24, size=shape, dtype=np.int64)
+    return (k.astype(np.float32) * np.float32(2.0 ** -23) - np.float32(1.0))
+
+
+def wide(g, shape):
+    s = np.where(g.integers(0, 2, size=shape) == 0, -1.0, 1.0)
+    e = g.integers(-12, 13, size=shape)
+    u = g.integers(0, 1 << 23, size=shape).astype(np.float64) * 2.0 ** -23
+    return (s * np.ldexp(1.0 + u, e)).astype(np.float32)
+
+
+def timed(fn, steps, warmup, stream):
+    """(ms per step from events, ms per core kernel launch, wall seconds)."""
     for _ in range(warmup):
-        ug.sgemm_dev("R", "N", "N", n, n, n, alpha, dA, n, dB, n, beta, dC, n, mode=mode_id, stream=stream)
+        fn()
     stream.synchronize()
     hip.synchronize()
     e0, e1 = hip.Event(), hip.Event()
+    ug.timing_enable(True)
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(steps):
-        ug.sgemm_dev("R", "N", "N", n, n, n, alpha, dA, n, dB, n, beta, dC, n, mode=mode_id, stream=stream)
+        fn()
     e1.record(stream)
     e1.synchronize()
     hip.synchronize()
     wall = time.perf_counter() - t0
+    n, core_ms = ug.timing_collect()
+    ug.timing_enable(False)
     ms = e0.elapsed_ms(e1) / steps
     e0.destroy()
     e1.destroy()
-    return ms, wall
+    return ms, (core_ms / n if n else None), wall
 
 
-def errors(mode_id, n, A, B, C0, dA, dB, dC, alpha, beta, stream, rows):
-    dC.upload(C0)
-    ug.sgemm_dev("R", "N", "N", n, n, n, alpha, dA, n, dB, n, beta, dC, n, mode=mode_id, stream=stream)
+def err_stats(got, exact, scale):
+    got = got.astype(np.float64)
+    d = np.abs(got - exact)
+    nz = exact != 0
+    rel = d[nz] / np.abs(exact[nz])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bn = np.where(scale > 0, d / scale, 0.0)
+    return {"max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean()),
+            "max_abs_err": float(d.max()), "rms_abs_err": float(np.sqrt(np.mean(d * d))),
+            "max_err_over_bound_scale": float(bn.max())}
+
+
+def sgemm_forms(n, A, B, C0, alpha, beta, forms, steps, warmup, stream, rows, tag):
+    dA, dB, dC = hip.DeviceArray.from_host(A), hip.DeviceArray.from_host(B), hip.DeviceArray.from_host(C0)
+    A64 = A[rows].astype(np.float64)
+    B64 = B.astype(np.float64)
+    exact = alpha * (A64 @ B64) + beta * C0[rows].astype(np.float64)
+    scale = abs(alpha) * (np.abs(A64) @ np.abs(B64)) + abs(beta) * np.abs(C0[rows].astype(np.float64))
+    del B64
+    flop = 2.0 * n ** 3
+    out = {}
+    for name in forms:
+        mid = ug.MODES[name]
+
+        def call():
+            ug.sgemm_dev("R", "N", "N", n, n, n, alpha, dA, n, dB, n, beta, dC, n, mode=mid, stream=stream)
+        try:
+            ms, core_ms, wall = timed(call, steps, warmup, stream)
+        except ug.UgemmError as e:
+            out[name] = {"error": str(e)}
+            continue
+        dC.upload(C0)
+        call()
+        stream.synchronize()
+        got = dC.download()[rows]
+        peak_key, mult = FORM_PEAK[name]
+        tf = flop / (ms * 1e-3) / 1e12
+        rec = {"tflops": round(tf, 3), "ms_per_call": round(ms, 4), "wall_s": round(wall, 3),
+               "core_kernel": CORE_KERNEL[name], "core_ms": round(core_ms, 4) if core_ms else None,
+               "core_tflops": round(flop / (core_ms * 1e-3) / 1e12, 3) if core_ms else None,
+               "frac_of_fp32_vector_peak": round(tf / PEAK["fp32_vector"], 4),
+               "mfma_utilisation": round(tf * mult / PEAK[peak_key], 4) if name != "fp32" else None}
+        rec.update(err_stats(got, exact, scale))
+        out[name] = rec
+        log(f"[{tag}] {name}: {tf:.1f} TFLOP/s ({ms:.3f} ms/call, core {core_ms:.3f} ms), "
+            f"max rel err {rec['max_rel_err']:.3g}, rms abs {rec['rms_abs_err']:.3g}")
+    for d in (dA, dB, dC):
+        d.free()
+    return out
+
+
+def config3(stream, steps, warmup):
+    """Tall-skinny and short-wide SGEMM, DGEMM 4096^3; normal(0,1); alpha=1, beta=0."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D03))
+    res = {}
+    big = g.standard_normal((65536, 4096), dtype=np.float32)
+    small = g.standard_normal((4096, 256), dtype=np.float32)
+    for (M, N, K, A, B, label) in ((65536, 256, 4096, big, small, "sgemm_65536x256x4096"),
+                                   (256, 65536, 4096, small.T.copy(), big.T.copy(), "sgemm_256x65536x4096")):
+        dA, dB, dC = hip.DeviceArray.from_host(A), hip.DeviceArray.from_host(B), hip.DeviceArray((M, N), np.float32)
+        rows = np.sort(np.random.Generator(np.random.PCG64(4)).choice(M, 32, replace=False))
+        exact = A[rows].astype(np.float64) @ B.astype(np.float64)
+        scale = np.abs(A[rows].astype(np.float64)) @ np.abs(B.astype(np.float64))
+        res[label] = {}
+        for name in ("fp32", "bf16x3", "bf16x6"):
+            mid = ug.MODES[name]
+
+            def call():
+                ug.sgemm_dev("R", "N", "N", M, N, K, 1.0, dA, K, dB, N, 0.0, dC, N, mode=mid, stream=stream)
+            ms, core_ms, _ = timed(call, steps, warmup, stream)
+            got = dC.download()[rows]
+            r = {"tflops": round(2.0 * M * N * K / (ms * 1e-3) / 1e12, 3), "ms_per_call": round(ms, 4)}
+            r.update(err_stats(got, exact, scale))
+            res[label][name] = r
+            log(f"[config3] {label} {name}: {r['tflops']:.1f} TFLOP/s")
+        for d in (dA, dB, dC):
+            d.free()
+    # DGEMM 4096^3
+    n = 4096
+    A = g.standard_normal((n, n))
+    B = g.standard_normal((n, n))
+    dA, dB = hip.DeviceArray.from_host(A), hip.DeviceArray.from_host(B)
+    dC, dC2 = hip.DeviceArray((n, n), np.float64), hip.DeviceArray((n, n), np.float64)
+
+    def call():
+        ug.dgemm_dev("R", "N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n, stream=stream)
+    ms, core_ms, _ = timed(call, max(3, steps // 2), 1, stream)
+    ug.dgemm_dev("R", "N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC2, n, stream=stream)
     stream.synchronize()
-    C = dC.download()
-    ref = alpha * (A[rows].astype(np.float64) @ B.astype(np.float64)) + beta * C0[rows].astype(np.float64)
-    got = C[rows].astype(np.float64)
-    nz = ref != 0
-    rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
-    absmax = float(np.abs(got - ref).max())
-    return float(rel.max()), float(rel.mean()), absmax
+    C1, C2 = dC.download(), dC2.download()
+    rows = np.sort(np.random.Generator(np.random.PCG64(5)).choice(n, 16, replace=False))
+    exact = A[rows] @ B      # FP64 BLAS; the bound below is relative to |A||B|
+    scale = np.abs(A[rows]) @ np.abs(B)
+    r = {"tflops": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 3), "ms_per_call": round(ms, 4),
+         "frac_of_fp64_peak": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12 / PEAK["fp64"], 4),
+         "bitwise_reproducible": bool(np.array_equal(C1.view(np.uint64), C2.view(np.uint64)))}
+    r.update(err_stats(C1[rows], exact, scale))
+    res["dgemm_4096"] = r
+    log(f"[config3] dgemm 4096^3: {r['tflops']:.1f} TFLOP/s, reproducible {r['bitwise_reproducible']}")
+    for d in (dA, dB, dC, dC2):
+        d.free()
+    return res
 
 
-def cpu_baseline(n, seconds_target=10.0):
-    """The reference's sgemm_avx (sgemm_avx256.h:392) on a row band of the
-    same workload: M = band rows of C, N = K = n, one thread."""
+def config4(stream, fp32_single_tf):
+    """512 uneven im2col-shaped FP32 problems: M = 32*U{1..16}, N = H*W with
+    H = W in {7, 14, 28, 56}, K = 9*16*U{1..32}; U[-1,1) inputs."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D04))
+    count = 512
+    Ms = 32 * g.integers(1, 17, size=count)
+    HW = g.choice([7, 14, 28, 56], size=count)
+    Ns = HW * HW
+    Ks = 144 * g.integers(1, 33, size=count)
+    flop = float(np.sum(2.0 * Ms * Ns * Ks))
+    # Operands are slices of two random pools (read-only, may overlap); C is disjoint per problem.
+    pa, pb = uniform(g, 4 << 20), uniform(g, 16 << 20)
+    dpa, dpb = hip.DeviceArray.from_host(pa), hip.DeviceArray.from_host(pb)
+    c_off = np.concatenate([[0], np.cumsum(((Ms * Ns + 3) // 4) * 4)])
+    dC = hip.DeviceArray(int(c_off[-1]), np.float32)
+    a_off = 4 * g.integers(0, ((4 << 20) - Ms * Ks) // 4)
+    b_off = 4 * g.integers(0, ((16 << 20) - Ks * Ns) // 4)
+    probs = [ug.Problem(b"N", b"N", int(Ms[i]), int(Ns[i]), int(Ks[i]), 1.0, 0.0, dpa.at(a_off[i]), int(Ks[i]),
+                        dpb.at(b_off[i]), int(Ns[i]), dC.at(c_off[i]), int(Ns[i])) for i in range(count)]
+    plan = ug.GroupedPlan(probs, capture=True)
+    tuned = plan.tune()
+    reps = 10
+    for _ in range(2):
+        plan.run(stream)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        plan.run(stream)           # one graph launch = one kernel
+    e1.record(stream)
+    e1.synchronize()
+    ms_group = e0.elapsed_ms(e1) / reps
+    # Correctness of a few problems (FP64 reference, FP32 bound).
+    worst = 0.0
+    for i in np.argsort(Ms * Ns * Ks)[:6]:
+        M, N, K = int(Ms[i]), int(Ns[i]), int(Ks[i])
+        A = pa[a_off[i]:a_off[i] + M * K].reshape(M, K).astype(np.float64)
+        B = pb[b_off[i]:b_off[i] + K * N].reshape(K, N).astype(np.float64)
+        got = dC.download()[c_off[i]:c_off[i] + M * N].reshape(M, N).astype(np.float64)
+        bound = 2 * (K + 1) * 2.0 ** -24 * (np.abs(A) @ np.abs(B))
+        worst = max(worst, float(np.max(np.abs(got - A @ B) / bound)))
+    # Baseline: loop of plain-kernel launches over the same problems.
+    loop_reps = 3
+
+    def loop():
+        for p in probs:
+            ug.sgemm_dev("R", "N", "N", p.M, p.N, p.K, 1.0, p.A, p.lda, p.B, p.ldb, 0.0, p.C, p.ldc, stream=stream)
+    loop()
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(loop_reps):
+        loop()
+    e1.record(stream)
+    e1.synchronize()
+    ms_loop = e0.elapsed_ms(e1) / loop_reps
+    plan.destroy()
+    tf_group = flop / (ms_group * 1e-3) / 1e12
+    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.variants(ug.MODE_FP32)[tuned],
+         "grouped_graph_ms": round(ms_group, 4), "grouped_tflops": round(tf_group, 3),
+         "gemms_per_s": round(count / (ms_group * 1e-3), 1),
+         "loop_plain_ms": round(ms_loop, 4), "loop_plain_tflops": round(flop / (ms_loop * 1e-3) / 1e12, 3),
+         "speedup_vs_loop": round(ms_loop / ms_group, 3),
+         "frac_of_single_large_gemm_rate": round(tf_group / fp32_single_tf, 4) if fp32_single_tf else None,
+         "checked_problems_worst_err_over_bound": worst}
+    log(f"[config4] grouped {tf_group:.1f} TFLOP/s ({ms_group:.3f} ms), loop {r['loop_plain_tflops']:.1f} TFLOP/s, "
+        f"{r['frac_of_single_large_gemm_rate']} of single-GEMM rate")
+    for d in (dpa, dpb, dC):
+        d.free()
+    e0.destroy()
+    e1.destroy()
+    return r
+
+
+def cpu_baseline(n, seconds_target):
+    """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
+    by the workload) on a row band of the n^3 workload, one thread."""
     try:
         from oracle import oracle as orc
-    except Exception:
-        return None
+    except Exception as e:  # noqa: BLE001
+        return {"error": f"oracle unavailable: {e}"}
     L = orc.ref_lib()
-    kind = "reference"
-    fn = None
     if L is not None:
-        fn = L.sgemm_avx
+        fn, kind, fname = L.sgemm_avx, "reference", "sgemm_avx (sgemm_avx256.h:392)"
     else:
-        kind = "port"
-        fn = orc.lib().oracle_sgemm_packed
+        fn, kind, fname = orc.lib().oracle_sgemm_packed, "port", "oracle_sgemm_packed (gemm_cpu.h blocking)"
     band = 256
     g = np.random.Generator(np.random.PCG64(7))
-    A = (g.random((band, n), dtype=np.float32) * 2 - 1).astype(np.float32)
-    B = (g.random((n, n), dtype=np.float32) * 2 - 1).astype(np.float32)
-    C = (g.random((band, n), dtype=np.float32) * 2 - 1).astype(np.float32)
+    A, B, C = uniform(g, (band, n)), uniform(g, (n, n)), uniform(g, (band, n))
     calls, t_total = 0, 0.0
     while t_total < seconds_target and calls < 64:
         t0 = time.perf_counter()
@@ -102,7 +285,7 @@ def cpu_baseline(n, seconds_target=10.0):
         t_total += time.perf_counter() - t0
         calls += 1
     tflops = 2.0 * band * n * n * calls / t_total / 1e12
-    cpu = platform.processor() or platform.machine()
+    cpu = platform.machine()
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -110,11 +293,19 @@ def cpu_baseline(n, seconds_target=10.0):
                 break
     except OSError:
         pass
-    fname = "sgemm_avx (sgemm_avx256.h:392)" if kind == "reference" else "oracle_sgemm_packed (gemm_cpu.h blocking)"
     return {"value": round(tflops, 6), "unit": "TFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"{fname}, R/N/N, M={band} rows x N=K={n} (a row band of the {n}^3 workload), "
-                      f"alpha=1.5 beta=0.5, {calls} calls in {t_total:.1f} s, 1 thread on {cpu}, "
-                      f"nproc={os.cpu_count()}"}
+            "sample": f"{fname}, R/N/N, alpha=1.5 beta=0.5, a {band}-row band of C (M={band}, N=K={n}) of the "
+                      f"{n}^3 workload, {calls} calls in {t_total:.1f} s, 1 thread of {cpu} (nproc={os.cpu_count()})"}
+
+
+def load_traffic(form):
+    """HBM bytes per launch of the headline core kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        return json.load(open(p)).get(form)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -123,66 +314,81 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
-    ap.add_argument("--modes", default="fp32,bf16x3,bf16x6,bf16x1,fp8")
+    ap.add_argument("--forms", default="bf16x3,fp32,bf16x6,bf16x1,fp8")
     ap.add_argument("--headline", default="bf16x3")
+    ap.add_argument("--quick", action="store_true", help="headline forms only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build is a single-GPU GEMM backend); "
-              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU: this backend is a single-GPU GEMM library (multi-GPU sharding is "
+              f"design-only); --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as one process on one GPU", file=sys.stderr)
         sys.exit(2)
 
     n = args.size
     alpha, beta = 1.5, 0.5
     ug.init(0, 0)
-    A, B, C0 = gen_inputs(n, 0x55474D4D02)
-    dA, dB, dC = hip.DeviceArray.from_host(A), hip.DeviceArray.from_host(B), hip.DeviceArray.from_host(C0)
     stream = hip.Stream()
+    g = np.random.Generator(np.random.PCG64(0x55474D4D02))
+    A, B, C0 = uniform(g, (n, n)), uniform(g, (n, n)), uniform(g, (n, n))
     rows = np.sort(np.random.Generator(np.random.PCG64(3)).choice(n, 64, replace=False))
-    flop = 2.0 * n * n * n
+    forms = [f for f in args.forms.split(",") if f]
+    res = sgemm_forms(n, A, B, C0, alpha, beta, forms, args.steps, args.warmup, stream, rows, "config2")
+    del A, B, C0
 
-    results = {}
-    for name in args.modes.split(","):
-        mid = ug.MODES[name]
-        try:
-            ms, wall = time_mode(mid, n, dA, dB, dC, alpha, beta, args.steps, args.warmup, stream)
-        except ug.UgemmError as e:
-            results[name] = {"error": str(e)}
-            continue
-        mx, mean, absmax = errors(mid, n, A, B, C0, dA, dB, dC, alpha, beta, stream, rows)
-        tf = flop / (ms * 1e-3) / 1e12
-        peak_key, mult = MODE_PEAK[name]
-        results[name] = {"tflops": round(tf, 3), "ms": round(ms, 4), "wall_s": round(wall, 3),
-                         "max_rel_err": mx, "mean_rel_err": mean, "max_abs_err": absmax,
-                         "frac_fp32_vector": round(tf / PEAK["fp32_vector"], 4),
-                         "frac_native_peak": round(tf * mult / PEAK[peak_key], 4), "native_peak": peak_key}
+    head = args.headline if "tflops" in res.get(args.headline, {}) else "fp32"
+    h = res[head]
+    peak_key, mult = FORM_PEAK[head]
+    flop = 2.0 * n ** 3
+    core_tf = h["core_tflops"] or h["tflops"]
+    roof = {"bound": "mfma", "kernel": h["core_kernel"], "achieved": core_tf,
+            "peak": PEAK[peak_key], "unit": "TFLOP/s", "frac": round(core_tf / PEAK[peak_key], 4),
+            "traffic": load_traffic(head),
+            "achieved_is": "algorithmic 2*M*N*K flop per launch / average core-kernel duration (HIP events)",
+            "mfma_issue_frac": round(core_tf * mult / PEAK[peak_key], 4),
+            "frac_of_fp32_vector_peak": round(core_tf / PEAK["fp32_vector"], 4)}
 
-    head = args.headline if args.headline in results and "tflops" in results[args.headline] else "fp32"
-    h = results[head]
-    peak_key, mult = MODE_PEAK[head]
-    roof = {"bound": "mfma" if "mfma" in peak_key else "fp32_vector" if False else "mfma" if head != "fp32" else "mfma",
-            "achieved": h["tflops"], "peak": PEAK[peak_key] if head != "fp32" else PEAK["fp32_vector"],
-            "unit": "TFLOP/s"}
-    if head == "fp32":
-        roof["bound"] = "mfma"  # compute-bound; peak is the FP32 vector rate
-    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
-    roof["traffic"] = None
-    if head != "fp32":
-        roof["mfma_utilisation"] = round(h["tflops"] * mult / PEAK[peak_key], 4)
     line = {
         "metric": "SGEMM TFLOP/s at 8192^3: plain FP32 vs BF16x3-MFMA at FP32-level error",
         "value": h["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": h["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (bf16 MFMA, split)" if head.startswith("bf16") else "f32",
-        "data": "synthetic U[-1,1], PCG64 seed 0x55474D4D02",
-        "config": {"workload": f"sgemm {n}x{n}x{n} row-major NN alpha=1.5 beta=0.5, headline form {head}",
+        "ms_per_step": h["ms_per_call"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (split bf16 on MFMA, f32 accumulate)" if head.startswith("bf16") else "f32",
+        "data": "synthetic: U[-1,1) 24-bit from PCG64 seed 0x55474D4D02, resident in HBM",
+        "config": {"workload": f"sgemm {n}x{n}x{n} row-major NN alpha={alpha} beta={beta}, form {head}; "
+                               "step = one GEMM call (operand split + MFMA core)",
                    "M": n, "N": n, "K": n, "layout": "NN", "form": head},
-        "forms": results,
+        "forms": res,
         "roofline": roof,
     }
-    if "fp32" in results and "tflops" in results["fp32"] and head != "fp32":
-        line["speedup_vs_plain_fp32"] = round(h["tflops"] / results["fp32"]["tflops"], 3)
+    if "fp32" in res and "tflops" in res["fp32"]:
+        line["speedup_vs_plain_fp32"] = {k: round(v["tflops"] / res["fp32"]["tflops"], 3)
+                                         for k, v in res.items() if "tflops" in v}
+        line["error_vs_plain_fp32"] = {k: round(v["rms_abs_err"] / res["fp32"]["rms_abs_err"], 3)
+                                       for k, v in res.items() if "rms_abs_err" in v}
+    if args.calibrate_copy:
+        nbytes = 1 << 30
+        src, dst = hip.DeviceArray(nbytes // 4, np.float32), hip.DeviceArray(nbytes // 4, np.float32)
+        hip.check(hip.lib().hipMemcpy(dst.ptr, src.ptr, nbytes, hip.D2D), "calibration copy")
+        hip.synchronize()
+        line["calibration_copy_bytes"] = nbytes
+        src.free()
+        dst.free()
+    if not args.quick:
+        line["config3"] = config3(stream, args.steps, args.warmup)
+        fp32_tf = res.get("fp32", {}).get("tflops")
+        line["config4_batched"] = config4(stream, fp32_tf)
+        g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
+        A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
+        C5 = np.zeros((n, n), np.float32)
+        line["config5_precision_ladder"] = sgemm_forms(n, A5, B5, C5, 1.0, 0.0, ["fp32", "bf16x3", "bf16x6",
+                                                                                  "bf16x1", "fp8"],
+                                                       max(5, args.steps // 4), 2, stream, rows, "config5")
+        del A5, B5, C5
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -140,6 +140,14 @@ int ugemm_hip_variant_count(int mode);
 const char *ugemm_hip_variant_name(int mode, int variant);
 const char *ugemm_hip_version(void);
 
+/* Launch timing for benchmarks (no counterpart in the reference): while
+ * enabled on the calling thread, the GEMM core kernel of every call (not the
+ * operand-preparation kernels) is bracketed by HIP events on its own stream.
+ * ugemm_hip_timing_collect waits for them, stores the summed milliseconds and
+ * returns how many launches were timed (then starts a new collection). */
+int ugemm_hip_timing_enable(int on);
+int ugemm_hip_timing_collect(double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

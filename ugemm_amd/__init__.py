@@ -82,6 +82,8 @@ def load(path=None):
         "ugemm_hip_variant_count": (i, [i]),
         "ugemm_hip_variant_name": (cs, [i, i]),
         "ugemm_hip_version": (cs, []),
+        "ugemm_hip_timing_enable": (i, [i]),
+        "ugemm_hip_timing_collect": (i, [ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -150,6 +152,20 @@ def dgemm_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC,
     rc = load().ugemm_hip_dgemm_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
                                     _ptr(dB), ldb, beta, _ptr(dC), ldc, s)
     return _raise_if(rc)
+
+
+def timing_enable(on=True):
+    """Bracket the core kernel of every following call (this thread) with HIP events."""
+    load().ugemm_hip_timing_enable(1 if on else 0)
+
+
+def timing_collect():
+    """(number of timed launches, summed milliseconds) since the last collect."""
+    ms = ctypes.c_double(0.0)
+    n = load().ugemm_hip_timing_collect(ctypes.byref(ms))
+    if n < 0:
+        _raise_if(n)
+    return n, ms.value
 
 
 def variants(mode):

@@ -139,6 +139,7 @@ int launch_cfg(const Gemm<double> &g, hipStream_t s) {
 #define UGEMM_L(TA, TB, VEC)                                                                                \
     hipLaunchKernelGGL((dgemm_mfma_kernel<CF, TA, TB, VEC>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, \
                        g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n)
+    timing_begin(s);
     if (vec) {
         if (!g.ta && !g.tb) UGEMM_L(false, false, 2);
         else if (!g.ta && g.tb) UGEMM_L(false, true, 2);
@@ -150,6 +151,7 @@ int launch_cfg(const Gemm<double> &g, hipStream_t s) {
         else if (g.ta && !g.tb) UGEMM_L(true, false, 1);
         else UGEMM_L(true, true, 1);
     }
+    timing_end(s);
 #undef UGEMM_L
     UGEMM_HIP_TRY(hipGetLastError());
     return UGEMM_OK;

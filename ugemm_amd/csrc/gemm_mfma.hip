@@ -415,8 +415,10 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
             attr = true;
         }
         if (!rc) {
+            timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
                                (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n);
+            timing_end(s);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = hip_fail(e, "bf16_nt_core launch");
         }

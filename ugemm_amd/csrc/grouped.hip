@@ -134,8 +134,10 @@ int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (p->grid == 0) return UGEMM_OK;
     const VariantOps ops = variant_ops(p->variant);
     void *args[] = {&p->d_probs, &p->d_tiles, &p->d_wg};
-    UGEMM_HIP_TRY(hipLaunchKernel(p->vec == 4 ? ops.kern_vec4 : ops.kern_vec1, dim3(p->grid), dim3(ops.nt), args, 0,
-                                  s));
+    timing_begin(s);
+    hipError_t e = hipLaunchKernel(p->vec == 4 ? ops.kern_vec4 : ops.kern_vec1, dim3(p->grid), dim3(ops.nt), args, 0, s);
+    timing_end(s);
+    if (e != hipSuccess) return hip_fail(e, "grouped launch");
     return UGEMM_OK;
 }
 

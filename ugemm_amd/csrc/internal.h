@@ -65,6 +65,13 @@ int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K);   // -1 if n
 int forced_variant(int mode);                                         // -1 if none
 void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms);
 
+// ---- launch timing (runtime.hip) ----
+// When enabled (ugemm_hip_timing_enable), the dominant kernel of every call
+// (the GEMM core, not the operand-preparation kernels) is bracketed by a pair
+// of HIP events recorded on its own stream; ugemm_hip_timing_collect sums them.
+void timing_begin(hipStream_t s);
+void timing_end(hipStream_t s);
+
 // ---- device scratch (runtime.hip) ----
 // Stream-ordered workspace for the operand-preparation kernels: allocated
 // from the device's default memory pool on stream s (which keeps freed

@@ -14,6 +14,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 namespace ugemm {
 
@@ -119,6 +121,37 @@ int stream_free(void *ptr, hipStream_t s) {
     hipError_t e = hipFreeAsync(ptr, s);
     if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(workspace)");
     return UGEMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch timing: a per-thread list of event pairs (events are reused).
+struct TimingState {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    size_t used = 0;
+};
+static thread_local TimingState t_timing;
+
+void timing_begin(hipStream_t s) {
+    TimingState &T = t_timing;
+    if (!T.on) return;
+    if (T.used == T.pairs.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            if (a) (void)hipEventDestroy(a);
+            T.on = false;
+            return;
+        }
+        T.pairs.emplace_back(a, b);
+    }
+    (void)hipEventRecord(T.pairs[T.used].first, s);
+}
+
+void timing_end(hipStream_t s) {
+    TimingState &T = t_timing;
+    if (!T.on || T.used >= T.pairs.size()) return;
+    (void)hipEventRecord(T.pairs[T.used].second, s);
+    ++T.used;
 }
 
 // ---------------------------------------------------------------------------
@@ -404,6 +437,30 @@ int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int 
     if (rc) return rc;
     rc = run_dgemm(g, -1, (hipStream_t)stream);
     return rc ? rc : ok();
+}
+
+int ugemm_hip_timing_enable(int on) {
+    TimingState &T = t_timing;
+    T.on = on != 0;
+    T.used = 0;
+    return UGEMM_OK;
+}
+
+int ugemm_hip_timing_collect(double *total_ms) {
+    TimingState &T = t_timing;
+    double sum = 0;
+    int n = 0;
+    for (size_t i = 0; i < T.used; ++i) {
+        if (hipEventSynchronize(T.pairs[i].second) != hipSuccess) return fail(UGEMM_ERR_HIP, "timing: sync");
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, T.pairs[i].first, T.pairs[i].second) != hipSuccess)
+            return fail(UGEMM_ERR_HIP, "timing: elapsed");
+        sum += ms;
+        ++n;
+    }
+    T.used = 0;
+    if (total_ms) *total_ms = sum;
+    return n;
 }
 
 int ugemm_hip_variant_count(int mode) {

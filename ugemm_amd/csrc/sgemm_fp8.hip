@@ -274,9 +274,11 @@ int run(const Gemm<float> &g, hipStream_t s) {
             attr = true;
         }
         if (!rc) {
+            timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
                                (int)Kp, Aq, (int)Mp, ea, Bq, (int)Np, eb, g.alpha, g.beta, g.C, g.ldc, tiles_m,
                                tiles_n);
+            timing_end(s);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = hip_fail(e, "fp8_scaled_core launch");
         }
