@@ -136,3 +136,50 @@ def test_grouped_argument_errors():
     arr = (ug.Problem * 1)(ok)
     assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_BF16X3, None) == ug.ERR_UNSUPPORTED
     assert L.ugemm_hip_sgemm_grouped(0, None, ug.MODE_FP32, None) == ug.OK
+
+
+@pytest.mark.parametrize("capture", [False, True])
+def test_grouped_fewer_tiles_than_queues(capture):
+    """Three one-tile problems: a grid of 3 workgroups over 8 queues, five of
+    them empty; every workgroup must still find its tile and exit."""
+    cases = [DevCase(Case("R", "N", "N", 5 + i, 7 + 2 * i, 9 + i, 1.5, 0.5, seed=300 + i)) for i in range(3)]
+    plan = ug.GroupedPlan([d.problem() for d in cases], capture=capture)
+    s = hip.Stream()
+    for _ in range(2):
+        for d in cases:
+            d.reset()
+        plan.run(s)
+        s.synchronize()
+        for d in cases:
+            ok, worst = d.check()
+            assert ok, worst
+    plan.destroy()
+    s.destroy()
+
+
+def test_grouped_more_tiles_than_workgroups_two_plans():
+    """More tiles than resident workgroups (each takes several, and queues are
+    drained by neighbours), with two plans alternating on one stream: each
+    plan's queue heads are cleared before each of its launches."""
+    ug.force_variant(ug.MODE_FP32, 1)     # 64 x 64 tiles: 16 per problem
+    try:
+        a = [DevCase(Case("R", "N", "N", 256, 256, 144, 1.0, 0.0, seed=400 + i)) for i in range(120)]
+        b = [DevCase(Case("R", "T", "N", 96, 200, 70, -1.0, 1.0, seed=600 + i, pad=1)) for i in range(20)]
+        pa = ug.GroupedPlan([d.problem() for d in a], capture=True)
+        pb = ug.GroupedPlan([d.problem() for d in b], capture=False)
+    finally:
+        ug.force_variant(ug.MODE_FP32, -1)
+    s = hip.Stream()
+    for r in range(2):
+        for d in a + b:
+            d.reset()
+        pa.run(s)
+        pb.run(s)
+        pa.run(s)     # beta == 0 for plan a: a second run gives the same C
+        s.synchronize()
+        for d in a + b:
+            ok, worst = d.check()
+            assert ok, (r, d.case.M, d.case.N, d.case.K, worst)
+    pa.destroy()
+    pb.destroy()
+    s.destroy()

@@ -43,12 +43,15 @@ struct Fp32Cfg {
 
 // One workgroup computes the C tile at (m0, n0) of g; shared by the plain and
 // the grouped (persistent) kernels. lds must hold 2*(BK*LDA + BK*LDB) floats.
+// aligned == false (with VEC == 4) reads A and B with scalar loads: the
+// grouped kernel mixes problems whose operands allow 128-bit loads with
+// problems whose operands do not.
 template <class CF, bool TA, bool TB, int VEC>
 __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K, const float alpha,
                                           const float *__restrict__ A, const long lda,
                                           const float *__restrict__ B, const long ldb, const float beta,
                                           float *__restrict__ C, const long ldc, const int m0, const int n0,
-                                          float *lds) {
+                                          float *lds, const bool aligned = true) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, TM = CF::TM, TN = CF::TN, NT = CF::NT;
     constexpr int LDA = CF::LDA, LDB = CF::LDB;
     float *As = lds;                    // [2][BK][LDA]
@@ -74,13 +77,13 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
         const int k0 = kt * BK;
         const float *a0 = TA ? A + (long)k0 * lda + m0 : A + (long)m0 * lda + k0;
         const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
-        if (interior && kt < kfull) {
+        if (interior && kt < kfull && aligned) {
             ra.template load<false>(a0, lda, t, 0, 0);
             rb.template load<false>(b0, ldb, t, 0, 0);
         } else {
             const int mr = M - m0, nr = N - n0, kr = K - k0;
-            ra.template load<true>(a0, lda, t, TA ? mr : kr, TA ? kr : mr);
-            rb.template load<true>(b0, ldb, t, TB ? kr : nr, TB ? nr : kr);
+            ra.template load<true>(a0, lda, t, TA ? mr : kr, TA ? kr : mr, aligned);
+            rb.template load<true>(b0, ldb, t, TB ? kr : nr, TB ? nr : kr, aligned);
         }
     };
     auto store_tile = [&](int buf) {

@@ -27,6 +27,8 @@
 // of A and the same k of column l%16 of B; D holds rows 4*(l/16)+i, column
 // l%16.  A and B use the same k assignment, so any order of the hardware's
 // internal sum pairs the right elements.
+#include <string>
+
 #include "internal.h"
 #include "kernel_common.h"
 
@@ -356,9 +358,14 @@ constexpr int kNumVariants = 4;
 
 template <class CF>
 const char *cfg_name() {
-    static char buf[48];
-    snprintf(buf, sizeof buf, "bf16x%d_nt_%dx%dx%d", CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN, CF::BK);
-    return buf;
+    // Formatted once per instantiation; function-local static initialisation is thread-safe.
+    static const std::string name = [] {
+        char buf[48];
+        snprintf(buf, sizeof buf, "bf16x%d_nt_%dx%dx%d", CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN,
+                 CF::BK);
+        return std::string(buf);
+    }();
+    return name.c_str();
 }
 
 int planes_of(int mode) {
@@ -407,13 +414,10 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     if (!rc) rc = prep_operand<NP>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, s);
     if (!rc) {
         auto kern = bf16_nt_core<CF>;
-        static bool attr = false;
-        if (!attr) {
-            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               CF::LDS_BYTES);
-            if (e != hipSuccess) rc = hip_fail(e, "hipFuncSetAttribute");
-            attr = true;
-        }
+        // Once per kernel instantiation (thread-safe static initialisation).
+        static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
+        if (attr != hipSuccess) rc = hip_fail(attr, "hipFuncSetAttribute");
         if (!rc) {
             timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
@@ -442,16 +446,13 @@ int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
 template <int NP>
 const char *name_np(int v) {
     using T = Table<NP>;
-    static const char *names[kNumVariants] = {nullptr};
-    if (!names[0]) {
-        static char b0[48], b1[48], b2[48], b3[48];
-        snprintf(b0, 48, "%s", cfg_name<typename T::V0>());
-        snprintf(b1, 48, "%s", cfg_name<typename T::V1>());
-        snprintf(b2, 48, "%s", cfg_name<typename T::V2>());
-        snprintf(b3, 48, "%s", cfg_name<typename T::V3>());
-        names[1] = b1; names[2] = b2; names[3] = b3; names[0] = b0;
+    switch (v) {
+        case 0: return cfg_name<typename T::V0>();
+        case 1: return cfg_name<typename T::V1>();
+        case 2: return cfg_name<typename T::V2>();
+        case 3: return cfg_name<typename T::V3>();
+        default: return nullptr;
     }
-    return names[v];
 }
 
 }  // namespace

@@ -2,22 +2,35 @@
 // persistent launch (optionally replayed as a hipGraph).
 //
 // Plan (host, once): every problem is cut into BM x BN tiles of one kernel
-// variant; each tile costs ~ceil(K/BK) + 2 k-steps.  Tiles are sorted by cost
-// and assigned longest-first to the least-loaded of G workgroups (G = CUs x
-// resident blocks per CU), giving each workgroup a static list.  Problem
-// descriptors, the tile table and the per-workgroup ranges are uploaded once;
-// the launch reads nothing else, so the captured graph replays identically
-// (no counters to reset, no atomics, deterministic results).
-// Kernel: workgroup b walks its tile range and runs the plain FP32 tile body
-// (fp32_tile.h) on each tile, dispatching on the tile's layout.
+// variant, listed in grouped order (8 tile-rows walked column by column, as
+// in tile_coords); each tile costs ~ceil(K/BK) + 2 k-steps.  Tiles are sorted
+// by cost, longest first (stably, so a problem's tiles stay together), and
+// dealt in chunks of W = G/8 consecutive tiles to 8 queues, chunk c to queue
+// c % 8 (G = CUs x resident blocks per CU, the persistent grid).
+// Run: a one-block kernel zeroes the 8 queue heads, then the persistent
+// kernel starts.  Workgroup b takes tiles from queue b % 8 -- the hardware's
+// round-robin dispatch puts block b on XCD b % 8, so one XCD's workgroups
+// take consecutive tiles of the same problems and share their A and B panels
+// in its L2 -- by an atomic increment of the queue head; when that queue is
+// empty it moves to the next one, and it exits after finding all 8 empty
+// (a queue never refills within a launch, so every workgroup reaches the
+// exit).  Taking tiles on demand balances the load whatever each tile really
+// costs: measured tile times vary by +-20% around the k-step model (edge
+// tiles, co-resident workgroups), which a static schedule cannot absorb.
+// Each tile is computed whole by one workgroup, so results do not depend on
+// which workgroup took it: replays are bitwise identical.  The heads are
+// cleared by a kernel, not a memset, so the pair can be captured in a graph.
+// Kernel body: the plain FP32 tile (fp32_tile.h), dispatched on the
+// problem's layout; 128-bit operand loads are used for the problems whose A
+// and B allow them.
+// A plan's launches must not overlap in time (the queue heads belong to the
+// plan): run one plan from one stream, or order the streams.
 // Used for convolution-as-GEMM batches (the conv -> GEMM shape mapping of
 // sgemm_ocl1.h:271-300: M = output channels, N = output pixels, K = k*k*Cin).
 #include "internal.h"
 #include "fp32_tile.h"
 
 #include <algorithm>
-#include <functional>
-#include <queue>
 #include <utility>
 #include <vector>
 
@@ -30,38 +43,61 @@ struct DevProblem {
     long lda, ldb, ldc;
     int M, N, K;          // K is 0 when alpha == 0 (A and B are then not read)
     float alpha, beta;
-    int layout;           // bit 0: A transposed, bit 1: B transposed
+    int layout;           // bit 0: A transposed, bit 1: B transposed, bit 2: A and B allow 128-bit loads
 };
 
 struct DevTile {
     int problem, m0, n0, pad;
 };
 
-template <class CF, int VEC>
-__global__ __launch_bounds__(CF::NT) void grouped_fp32_kernel(const DevProblem *__restrict__ probs,
-                                                              const DevTile *__restrict__ tiles,
-                                                              const int *__restrict__ wg_begin) {
+constexpr int kQueues = 8;   // one per XCD
+
+__global__ void grouped_reset_heads(int *heads) {
+    if (threadIdx.x < kQueues) heads[threadIdx.x] = 0;
+}
+
+// At least 3 waves per SIMD (<= 168 VGPRs): the layout switch would
+// otherwise push the 128 x 128 variant just past that limit.
+template <class CF>
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) void grouped_fp32_kernel(
+    const DevProblem *__restrict__ probs, const DevTile *__restrict__ tiles, const int *__restrict__ q_begin,
+    int *heads) {
     __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
-    const int t0 = wg_begin[blockIdx.x], t1 = wg_begin[blockIdx.x + 1];
-    for (int i = t0; i < t1; ++i) {
+    __shared__ int s_tile;
+    int q = blockIdx.x % kQueues;
+    for (int empty = 0; empty < kQueues;) {
+        if (threadIdx.x == 0) {
+            const int base = q_begin[q], size = q_begin[q + 1] - base;
+            const int i = size > 0 ? atomicAdd(&heads[q], 1) : 0;
+            s_tile = i < size ? base + i : -1;
+        }
+        __syncthreads();
+        const int i = s_tile;
+        __syncthreads();   // every thread has read s_tile before it is written again
+        if (i < 0) {
+            q = (q + 1) % kQueues;
+            ++empty;
+            continue;
+        }
         const DevTile tl = tiles[i];
         const DevProblem p = probs[tl.problem];
-        switch (p.layout) {
+        const bool aligned = (p.layout & 4) != 0;
+        switch (p.layout & 3) {
             case 0:
-                fp32_tile<CF, false, false, VEC>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                                 tl.m0, tl.n0, lds);
+                fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
+                                               tl.m0, tl.n0, lds, aligned);
                 break;
             case 1:
-                fp32_tile<CF, true, false, VEC>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                                tl.m0, tl.n0, lds);
+                fp32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
+                                              tl.m0, tl.n0, lds, aligned);
                 break;
             case 2:
-                fp32_tile<CF, false, true, VEC>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                                tl.m0, tl.n0, lds);
+                fp32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
+                                              tl.m0, tl.n0, lds, aligned);
                 break;
             default:
-                fp32_tile<CF, true, true, VEC>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                               tl.m0, tl.n0, lds);
+                fp32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
+                                             tl.m0, tl.n0, lds, aligned);
                 break;
         }
         __syncthreads();   // the next tile's first LDS stage overwrites this one's
@@ -76,14 +112,13 @@ constexpr int kNumVariants = 4;
 
 struct VariantOps {
     int bm, bn, bk;
-    const void *kern_vec4, *kern_vec1;
+    const void *kern;
     int nt;
 };
 
 template <class CF>
 VariantOps ops_of() {
-    return VariantOps{CF::BM, CF::BN, CF::BK, (const void *)grouped_fp32_kernel<CF, 4>,
-                      (const void *)grouped_fp32_kernel<CF, 1>, CF::NT};
+    return VariantOps{CF::BM, CF::BN, CF::BK, (const void *)grouped_fp32_kernel<CF>, CF::NT};
 }
 
 VariantOps variant_ops(int v) {
@@ -105,12 +140,12 @@ struct ugemm_grouped_plan {
     double flops = 0;
     int mode = UGEMM_MODE_FP32;
     int variant = 0;
-    int vec = 4;
     int grid = 0;
     int ntiles = 0;
     DevProblem *d_probs = nullptr;
     DevTile *d_tiles = nullptr;
-    int *d_wg = nullptr;
+    int *d_queues = nullptr;   // kQueues + 1 queue starts
+    int *d_heads = nullptr;    // kQueues queue heads (zeroed before every launch)
     bool capture = false;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -125,23 +160,28 @@ void release_schedule(ugemm_grouped_plan *p) {
     p->exec = nullptr;
     p->graph = nullptr;
     hipFree(p->d_tiles);
-    hipFree(p->d_wg);
+    hipFree(p->d_queues);
+    hipFree(p->d_heads);
     p->d_tiles = nullptr;
-    p->d_wg = nullptr;
+    p->d_queues = nullptr;
+    p->d_heads = nullptr;
 }
 
 int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (p->grid == 0) return UGEMM_OK;
     const VariantOps ops = variant_ops(p->variant);
-    void *args[] = {&p->d_probs, &p->d_tiles, &p->d_wg};
+    hipLaunchKernelGGL(grouped_reset_heads, dim3(1), dim3(64), 0, s, p->d_heads);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "grouped queue reset");
+    void *args[] = {&p->d_probs, &p->d_tiles, &p->d_queues, &p->d_heads};
     timing_begin(s);
-    hipError_t e = hipLaunchKernel(p->vec == 4 ? ops.kern_vec4 : ops.kern_vec1, dim3(p->grid), dim3(ops.nt), args, 0, s);
+    e = hipLaunchKernel(ops.kern, dim3(p->grid), dim3(ops.nt), args, 0, s);
     timing_end(s);
     if (e != hipSuccess) return hip_fail(e, "grouped launch");
     return UGEMM_OK;
 }
 
-// Builds the tile table and the static longest-first schedule for variant v,
+// Builds the tile table and the per-XCD tile queues for variant v,
 // uploads it, and (if requested) captures the launch in a graph.
 int build_schedule(ugemm_grouped_plan *p, int v) {
     release_schedule(p);
@@ -157,42 +197,42 @@ int build_schedule(ugemm_grouped_plan *p, int v) {
         if (q.M == 0 || q.N == 0) continue;
         if (q.K == 0 && q.beta == 1.0f) continue;   // nothing to do
         const long cost = (q.K + ops.bk - 1) / ops.bk + 2;
-        for (int m0 = 0; m0 < q.M; m0 += ops.bm)
-            for (int n0 = 0; n0 < q.N; n0 += ops.bn) tiles.push_back(HostTile{DevTile{i, m0, n0, 0}, cost});
+        // Grouped order: GROUP_M tile-rows at a time, walked column by column.
+        constexpr int GROUP_M = 8;
+        const int tiles_m = (q.M + ops.bm - 1) / ops.bm, tiles_n = (q.N + ops.bn - 1) / ops.bn;
+        for (int g0 = 0; g0 < tiles_m; g0 += GROUP_M) {
+            const int g1 = std::min(tiles_m, g0 + GROUP_M);
+            for (int tn = 0; tn < tiles_n; ++tn)
+                for (int tm = g0; tm < g1; ++tm)
+                    tiles.push_back(HostTile{DevTile{i, tm * ops.bm, tn * ops.bn, 0}, cost});
+        }
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile &a, const HostTile &b) { return a.cost > b.cost; });
     int per_cu = 1;
-    const void *kern = p->vec == 4 ? ops.kern_vec4 : ops.kern_vec1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, ops.nt, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ops.kern, ops.nt, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     const long slots = (long)device_cu_count() * per_cu;
     const int G = (int)std::min<long>((long)tiles.size(), slots);
     p->grid = G;
     p->ntiles = (int)tiles.size();
     if (G == 0) return UGEMM_OK;
-    // Greedy longest-processing-time assignment.
-    std::vector<std::vector<int>> lists(G);
-    typedef std::pair<long, int> Load;
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (int g = 0; g < G; ++g) heap.push(Load(0, g));
-    for (int i = 0; i < (int)tiles.size(); ++i) {
-        Load l = heap.top();
-        heap.pop();
-        lists[l.second].push_back(i);
-        heap.push(Load(l.first + tiles[i].cost, l.second));
-    }
+    // Chunks of W consecutive (sorted) tiles, chunk c to queue c % kQueues.
+    const long ntiles = (long)tiles.size();
+    const long W = std::max<long>(1, G / kQueues);
     std::vector<DevTile> flat;
-    std::vector<int> begin(G + 1, 0);
+    std::vector<int> qbegin(kQueues + 1, 0);
     flat.reserve(tiles.size());
-    for (int g = 0; g < G; ++g) {
-        begin[g] = (int)flat.size();
-        for (int i : lists[g]) flat.push_back(tiles[i].t);
+    for (int q = 0; q < kQueues; ++q) {
+        qbegin[q] = (int)flat.size();
+        for (long c0 = q * W; c0 < ntiles; c0 += kQueues * W)
+            for (long j = c0; j < std::min(ntiles, c0 + W); ++j) flat.push_back(tiles[j].t);
     }
-    begin[G] = (int)flat.size();
+    qbegin[kQueues] = (int)flat.size();
     UGEMM_HIP_TRY(hipMalloc(&p->d_tiles, flat.size() * sizeof(DevTile)));
-    UGEMM_HIP_TRY(hipMalloc(&p->d_wg, begin.size() * sizeof(int)));
+    UGEMM_HIP_TRY(hipMalloc(&p->d_queues, qbegin.size() * sizeof(int)));
+    UGEMM_HIP_TRY(hipMalloc(&p->d_heads, kQueues * sizeof(int)));
     UGEMM_HIP_TRY(hipMemcpy(p->d_tiles, flat.data(), flat.size() * sizeof(DevTile), hipMemcpyHostToDevice));
-    UGEMM_HIP_TRY(hipMemcpy(p->d_wg, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice));
+    UGEMM_HIP_TRY(hipMemcpy(p->d_queues, qbegin.data(), qbegin.size() * sizeof(int), hipMemcpyHostToDevice));
     if (p->capture) {
         UGEMM_HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
         int rc = launch_plan_kernel(p, p->cap_stream);
@@ -241,7 +281,6 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
     if (rc) return rc;
     ugemm_grouped_plan *p = new ugemm_grouped_plan();
     p->mode = mode;
-    bool vec = true;
     for (int i = 0; i < count; ++i) {
         const ugemm_problem_t &q = problems[i];
         bool ta, tb;
@@ -263,14 +302,12 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
             delete p;
             return fail(UGEMM_ERR_ARG, "grouped: problem %d: NULL operand", i);
         }
+        const bool vec = ugemm_aligned16(q.A) && ugemm_aligned16(q.B) && q.lda % 4 == 0 && q.ldb % 4 == 0;
         DevProblem d{q.A, q.B, q.C, q.lda, q.ldb, q.ldc, q.M, q.N, reads_ab ? q.K : 0, q.alpha, q.beta,
-                     (ta ? 1 : 0) | (tb ? 2 : 0)};
-        if (reads_ab)
-            vec = vec && ugemm_aligned16(q.A) && ugemm_aligned16(q.B) && q.lda % 4 == 0 && q.ldb % 4 == 0;
+                     (ta ? 1 : 0) | (tb ? 2 : 0) | (vec ? 4 : 0)};
         p->probs.push_back(d);
         p->flops += 2.0 * q.M * q.N * q.K;
     }
-    p->vec = vec ? 4 : 1;
     p->capture = capture != 0;
     hipError_t e;
     if ((e = hipMalloc(&p->d_probs, std::max<size_t>(1, p->probs.size()) * sizeof(DevProblem))) != hipSuccess ||

@@ -59,8 +59,11 @@ struct TileRegs {
 
     // base points at the tile origin; element (c, s) is base[s * ld + c].
     // cmax / smax: valid extents from the origin (only used when CHECK).
+    // vec_ok == false (CHECK only): the operand is not 16-byte aligned, every
+    // chunk is read with scalar loads.
     template <bool CHECK>
-    __device__ __forceinline__ void load(const T *__restrict__ base, long ld, int t, int cmax, int smax) {
+    __device__ __forceinline__ void load(const T *__restrict__ base, long ld, int t, int cmax, int smax,
+                                         bool vec_ok = true) {
 #pragma unroll
         for (int i = 0; i < NCHUNK; ++i) {
             const int f = t + i * NT;
@@ -71,7 +74,7 @@ struct TileRegs {
                 load_vec(i, p);
             } else {
                 const bool srow = s < smax;
-                if (srow && c + VEC <= cmax) {
+                if (vec_ok && srow && c + VEC <= cmax) {
                     load_vec(i, p);
                 } else {
 #pragma unroll

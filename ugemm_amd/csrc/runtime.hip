@@ -326,17 +326,17 @@ static int run_host(const Gemm<T> &h, int mode) {
 }
 
 static int env_mode() {
-    static int mode = -2;
-    if (mode == -2) {
+    // Read once (thread-safe static initialisation).
+    static const int mode = [] {
         const char *e = getenv("UGEMM_HIP_MODE");
-        mode = UGEMM_MODE_FP32;
         if (e) {
-            if (!strcmp(e, "bf16x3")) mode = UGEMM_MODE_BF16X3;
-            else if (!strcmp(e, "bf16x1")) mode = UGEMM_MODE_BF16X1;
-            else if (!strcmp(e, "bf16x6")) mode = UGEMM_MODE_BF16X6;
-            else if (!strcmp(e, "fp8")) mode = UGEMM_MODE_FP8_BLOCK;
+            if (!strcmp(e, "bf16x3")) return (int)UGEMM_MODE_BF16X3;
+            if (!strcmp(e, "bf16x1")) return (int)UGEMM_MODE_BF16X1;
+            if (!strcmp(e, "bf16x6")) return (int)UGEMM_MODE_BF16X6;
+            if (!strcmp(e, "fp8")) return (int)UGEMM_MODE_FP8_BLOCK;
         }
-    }
+        return (int)UGEMM_MODE_FP32;
+    }();
     return mode;
 }
 

@@ -266,13 +266,10 @@ int run(const Gemm<float> &g, hipStream_t s) {
     if (!rc) rc = prep(g.B, g.ldb, g.tb, g.N, g.K, Bq, (int)Np, (int)Kp, eb, s);
     if (!rc) {
         auto kern = fp8_scaled_core<CF>;
-        static bool attr = false;
-        if (!attr) {
-            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               CF::LDS_BYTES);
-            if (e != hipSuccess) rc = hip_fail(e, "hipFuncSetAttribute");
-            attr = true;
-        }
+        // Once per kernel instantiation (thread-safe static initialisation).
+        static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
+        if (attr != hipSuccess) rc = hip_fail(attr, "hipFuncSetAttribute");
         if (!rc) {
             timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
