@@ -4,8 +4,8 @@
 Headline workload: SGEMM M = N = K = 8192, row-major NN, alpha = 1.5,
 beta = 0.5, A, B, C ~ U[-1, 1) (24-bit, seeded), inputs resident in HBM.
 A "step" is one call of the headline form, the split-BF16 x3 SGEMM on MFMA;
-the plain FP32 kernel and the other forms (BF16x6, BF16x1, FP8 block-scaled)
-are timed the same way beside it.  Timing: W untimed warm-up calls, then K
+the plain FP32 kernel and the other forms (FP16x3 with per-row scales,
+BF16x6, BF16x1, FP8 block-scaled) are timed the same way beside it.  Timing: W untimed warm-up calls, then K
 calls bracketed by device synchronisation and HIP events on the launch stream.
 The dominant kernel's own average duration (the MFMA core, without the
 operand-preparation kernels) comes from the library's event hook
@@ -37,13 +37,15 @@ import ugemm_amd as ug  # noqa: E402
 from ugemm_amd import hip  # noqa: E402
 
 # MI355X published dense peaks (TFLOP/s) and HBM bandwidth.
-PEAK = {"fp32_vector": 157.3, "bf16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}
+PEAK = {"fp32_vector": 157.3, "bf16_mfma": 2516.6, "fp16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}
 HBM_GBPS = 8000.0
 # Form -> (peak used for roofline.frac, MFMA products per useful product)
 FORM_PEAK = {"fp32": ("fp32_vector", 1), "bf16x3": ("bf16_mfma", 3), "bf16x6": ("bf16_mfma", 6),
+             "fp16x3": ("fp16_mfma", 3),
              "bf16x1": ("bf16_mfma", 1), "fp8": ("fp8_mfma", 1)}
-CORE_KERNEL = {"fp32": "sgemm_fp32_kernel", "bf16x3": "bf16_nt_core", "bf16x6": "bf16_nt_core",
-               "bf16x1": "bf16_nt_core", "fp8": "fp8_scaled_core"}
+CORE_KERNEL = {"fp32": "sgemm_fp32_kernel", "bf16x3": "split_nt_core", "bf16x6": "split_nt_core",
+               "fp16x3": "split_nt_core",
+               "bf16x1": "split_nt_core", "fp8": "fp8_scaled_core"}
 
 
 def log(*a):
@@ -150,7 +152,7 @@ def config3(stream, steps, warmup):
         exact = A[rows].astype(np.float64) @ B.astype(np.float64)
         scale = np.abs(A[rows].astype(np.float64)) @ np.abs(B.astype(np.float64))
         res[label] = {}
-        for name in ("fp32", "bf16x3", "bf16x6"):
+        for name in ("fp32", "bf16x3", "fp16x3", "bf16x6"):
             mid = ug.MODES[name]
 
             def call():
@@ -314,7 +316,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
-    ap.add_argument("--forms", default="bf16x3,fp32,bf16x6,bf16x1,fp8")
+    ap.add_argument("--forms", default="bf16x3,fp32,fp16x3,bf16x6,bf16x1,fp8")
     ap.add_argument("--headline", default="bf16x3")
     ap.add_argument("--quick", action="store_true", help="headline forms only")
     ap.add_argument("--no-cpu", action="store_true")
@@ -370,6 +372,16 @@ def main():
                                          for k, v in res.items() if "tflops" in v}
         line["error_vs_plain_fp32"] = {k: round(v["rms_abs_err"] / res["fp32"]["rms_abs_err"], 3)
                                        for k, v in res.items() if "rms_abs_err" in v}
+        # The throughput/accuracy target: >= 3x plain FP32 TFLOP/s with max and
+        # mean relative error vs FP64 no worse than plain FP32's.
+        p = res["fp32"]
+        line["target_3x_at_fp32_error"] = {
+            k: {"speedup": round(v["tflops"] / p["tflops"], 3),
+                "max_rel_err_ratio": round(v["max_rel_err"] / p["max_rel_err"], 3),
+                "mean_rel_err_ratio": round(v["mean_rel_err"] / p["mean_rel_err"], 3),
+                "met": bool(v["tflops"] >= 3 * p["tflops"] and v["max_rel_err"] <= p["max_rel_err"]
+                            and v["mean_rel_err"] <= p["mean_rel_err"])}
+            for k, v in res.items() if k != "fp32" and "tflops" in v and "max_rel_err" in v}
     if args.calibrate_copy:
         nbytes = 1 << 30
         src, dst = hip.DeviceArray(nbytes // 4, np.float32), hip.DeviceArray(nbytes // 4, np.float32)
@@ -385,8 +397,8 @@ def main():
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)
-        line["config5_precision_ladder"] = sgemm_forms(n, A5, B5, C5, 1.0, 0.0, ["fp32", "bf16x3", "bf16x6",
-                                                                                  "bf16x1", "fp8"],
+        line["config5_precision_ladder"] = sgemm_forms(n, A5, B5, C5, 1.0, 0.0, ["fp32", "bf16x3", "fp16x3",
+                                                                                  "bf16x6", "bf16x1", "fp8"],
                                                        max(5, args.steps // 4), 2, stream, rows, "config5")
         del A5, B5, C5
     if not args.no_cpu:

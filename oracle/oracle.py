@@ -277,3 +277,43 @@ def fp8_block_dequant(X, block=128):
             e = block_exponent(float(np.max(np.abs(t))) if t.size else 0.0)
             out[r0:r0 + block, k0:k0 + block] = quantize_e4m3(t / 2.0 ** e) * 2.0 ** e
     return out
+
+
+# ---------------------------------------------------------------------------
+# FP16X3: two-way fp16 split with a power-of-two scale per row of op(A) and
+# per column of op(B).  The reference has no such path; this restates the
+# build's own rule so that the form can be checked element by element.
+# Pinned (tests/test_oracle.py) by Python's struct IEEE-half codec, exact
+# small-integer inputs and the split's reconstruction bound.
+
+def fp16_row_exponent(amax):
+    """e such that a row with largest magnitude amax, scaled by 2^-e, lies in
+    (-2^15, 2^15); -15 for 0, inf and NaN."""
+    amax = float(amax)
+    if amax == 0.0 or not np.isfinite(amax):
+        return -15
+    return int(np.frexp(amax)[1]) - 15
+
+
+def fp16_split_rows(X):
+    """X (rows x K, logical, float32 values) -> (hi, lo, e): float64 arrays of
+    the fp16 planes of X_r * 2^-e_r (round to nearest even, subnormals kept)
+    and the per-row exponents."""
+    X = np.asarray(X, np.float32).astype(np.float64)
+    e = np.array([fp16_row_exponent(np.max(np.abs(r))) if r.size else -15 for r in X], np.int64)
+    Xs = np.ldexp(X, -e[:, None]).astype(np.float32)          # exact: only the exponent changes
+    with np.errstate(over="ignore", invalid="ignore"):
+        hi = Xs.astype(np.float16)
+        r = Xs - hi.astype(np.float32)                          # exact in float32
+        r = np.where(np.isfinite(hi), r, np.float32(0.0))
+        lo = r.astype(np.float16)
+    return hi.astype(np.float64), lo.astype(np.float64), e
+
+
+def fp16x3_product(Aop, Bop):
+    """The value FP16X3 computes for op(A) @ op(B) before alpha/beta, in
+    float64: 2^(e_m + e_n) * (Ahi Bhi + Ahi Blo + Alo Bhi)."""
+    ah, al, ea = fp16_split_rows(Aop)
+    bh, bl, eb = fp16_split_rows(np.asarray(Bop).T)
+    P = ah @ bh.T + ah @ bl.T + al @ bh.T
+    return np.ldexp(P, ea[:, None] + eb[None, :])

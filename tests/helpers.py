@@ -10,7 +10,8 @@ from oracle import oracle as orc
 #   bf16x3: dropped lo*lo and split residuals, |r| <= 2^-16 |x|  -> 3 * 2^-16 (+ slack)
 #   bf16x6: dropped terms of order 2^-24                         -> 8 * 2^-24
 #   bf16x1: both operands rounded to 8 significant bits         -> 2^-7 + 2^-16 (+ slack)
-MODE_EXTRA = {0: 0.0, 1: 2.0 ** -14, 2: 2.0 ** -6, 4: 2.0 ** -21}
+#   fp16x3: fp16 split residuals and lo*lo, each <= 2^-22 |a||b|  -> 3 * 2^-22 (+ slack)
+MODE_EXTRA = {0: 0.0, 1: 2.0 ** -14, 2: 2.0 ** -6, 4: 2.0 ** -21, 5: 2.0 ** -19}
 
 
 class Case:

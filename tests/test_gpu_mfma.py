@@ -1,8 +1,10 @@
-"""Parity of the split-BF16 MFMA forms (BF16X1, BF16X3, BF16X6) with the oracle.
+"""Parity of the split MFMA forms (BF16X1, BF16X3, BF16X6, FP16X3) with the oracle.
 
 Acceptance per element: |C - C_exact| <= (2(K+1) eps32 + extra) * scale with
-extra = 2^-6 (x1), 2^-14 (x3), 2^-21 (x6) -- the per-product error of the
-dropped split terms (see tests/helpers.py)."""
+extra = 2^-6 (x1), 2^-14 (x3), 2^-21 (x6), 2^-19 (fp16x3) -- the per-product
+error of the dropped split terms (see tests/helpers.py).  FP16X3 is also
+compared with its exact emulation (oracle.fp16x3_product), where only the
+FP32 accumulation may differ."""
 import zlib
 
 import numpy as np
@@ -15,7 +17,7 @@ from tests.helpers import Case
 
 pytestmark = pytest.mark.gpu
 
-MODES = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6]
+MODES = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3]
 COMBOS = [(m, a, b) for m in "RC" for a in "NT" for b in "NT"]
 SHAPES = [(1, 1, 1), (7, 5, 9), (33, 65, 31), (129, 127, 65), (200, 300, 257)]
 
@@ -67,17 +69,19 @@ def test_split_leading_dimensions_and_nan_c(mode, pad):
 
 
 def test_split_accuracy_ordering():
-    # K = 4096, U[-1,1]: rms error vs exact -- bf16x6 at or below plain FP32,
-    # bf16x3 within a small factor of it, bf16x1 far above (8-bit operands).
+    # K = 4096, U[-1,1]: rms error vs exact -- bf16x6 and fp16x3 at or below
+    # plain FP32, bf16x3 within a small factor of it, bf16x1 far above (8-bit
+    # operands).
     M = N = 256
     K = 4096
     case = Case("R", "N", "N", M, N, K, 1.0, 0.0, seed=77)
     exact, _ = case.exact()
     rms = {}
-    for mode in (ug.MODE_FP32, ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6):
+    for mode in (ug.MODE_FP32, ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3):
         C = run_host(case, mode)
         rms[mode] = float(np.sqrt(np.mean((C.astype(np.float64) - exact) ** 2)))
     assert rms[ug.MODE_BF16X6] <= rms[ug.MODE_FP32], rms
+    assert rms[ug.MODE_FP16X3] <= rms[ug.MODE_FP32], rms
     assert rms[ug.MODE_BF16X3] <= 4 * rms[ug.MODE_FP32], rms
     assert rms[ug.MODE_BF16X1] > 100 * rms[ug.MODE_FP32], rms
 
@@ -111,3 +115,38 @@ def test_split_large_sampled(mode):
     from tests.helpers import MODE_EXTRA
     ok, worst = orc.check_bound(C[rows, :N], exact, scale, orc.tolerance(K, extra=MODE_EXTRA[mode]))
     assert ok, f"worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("dist", ["uniform", "normal", "wide"])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_fp16x3_matches_emulation(dist, ta, tb):
+    """GPU FP16X3 vs the exact emulation of its split and scales: the two
+    differ only by FP32 accumulation, so the plain FP32 bound applies; wide
+    data (entries sign*2^e*(1+u), e in [-12, 12]) exercises the row scales."""
+    M, N, K = 96, 80, 300
+    case = Case("R", ta, tb, M, N, K, 1.0, 0.0, seed=zlib.crc32(f"f16{dist}{ta}{tb}".encode()), dist=dist)
+    C = run_host(case, ug.MODE_FP16X3)
+    Aop = orc.op_matrix("R", ta, case.A, M, K)
+    Bop = orc.op_matrix("R", tb, case.B, K, N)
+    emu = orc.fp16x3_product(Aop, Bop)
+    scale = np.abs(Aop.astype(np.float64)) @ np.abs(Bop.astype(np.float64))
+    ok, worst = orc.check_bound(C[:, :N], emu, scale, orc.tolerance(K))
+    assert ok, f"worst/bound {worst:.3g}"
+
+
+def test_fp16x3_row_scales_extreme_rows():
+    """Rows and columns of very different magnitude (2^-60 .. 2^60) and an
+    all-zero row: each row's scale brings it into fp16 range independently."""
+    M, N, K = 8, 6, 40
+    g = orc.rng(93)
+    A = orc.uniform(g, (M, K)) * np.ldexp(np.float32(1.0), np.arange(-60, 60, 15))[:, None].astype(np.float32)
+    A[5] = 0.0
+    B = orc.uniform(g, (K, N)) * np.ldexp(np.float32(1.0), np.array([-50, -20, 0, 10, 30, 50]))[None, :].astype(np.float32)
+    A = A.astype(np.float32); B = B.astype(np.float32)
+    C = np.zeros((M, N), np.float32)
+    ug.sgemm("R", "N", "N", M, N, K, 1.0, A, K, B, N, 0.0, C, N, mode=ug.MODE_FP16X3)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    ok, worst = orc.check_bound(C, exact, scale, orc.tolerance(K, extra=2.0 ** -19))
+    assert ok, worst
+    assert np.all(C[5] == 0.0)

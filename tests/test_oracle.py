@@ -258,3 +258,55 @@ def test_fp8_block_dequant_uses_one_scale_per_block():
     D = orc.fp8_block_dequant(X)
     assert D[0, 0] == 1792.0 and D[0, 1] == 5.0
     assert D[129, 200] == 1.0 and D[129, 201] == orc.quantize_e4m3(0.3 * 256) / 256
+
+
+# ---------------------------------------------------------------------------
+# FP16X3 emulation.
+
+def _half_rne(x):
+    """IEEE binary16 rounding of a float by Python's struct codec (an
+    implementation independent of numpy's casts)."""
+    import struct
+    return struct.unpack("<e", struct.pack("<e", float(x)))[0]
+
+
+def test_fp16_split_matches_struct_half_and_reconstructs():
+    g = orc.rng(91)
+    X = orc.wide_range(g, (6, 64), emin=-20, emax=20)
+    X[0, :3] = [1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, -(2.0 ** 14 + 2.0 ** 3)]   # ties to even
+    hi, lo, e = orc.fp16_split_rows(X)
+    for r in range(X.shape[0]):
+        xs = np.ldexp(X[r].astype(np.float64), -e[r])
+        assert np.max(np.abs(xs)) < 2.0 ** 15 and np.max(np.abs(xs)) >= 2.0 ** 14
+        for k in range(0, X.shape[1], 7):
+            h = _half_rne(xs[k])
+            assert hi[r, k] == h
+            assert lo[r, k] == _half_rne(np.float32(xs[k] - h))
+        # hi + lo within 2^-22 relative (or the fp16 subnormal step 2^-25 absolute)
+        err = np.abs(xs - hi[r] - lo[r])
+        assert np.all(err <= np.maximum(2.0 ** -22 * np.abs(xs), 2.0 ** -25))
+
+
+def test_fp16_row_exponent_special_cases():
+    assert orc.fp16_row_exponent(0.0) == -15
+    assert orc.fp16_row_exponent(np.inf) == -15 and orc.fp16_row_exponent(np.nan) == -15
+    assert orc.fp16_row_exponent(1.0) == -14            # 1 * 2^14 in [2^14, 2^15)
+    assert orc.fp16_row_exponent(2.0 ** 15 - 1) == 0
+    assert orc.fp16_row_exponent(1e-30) == int(np.frexp(1e-30)[1]) - 15
+
+
+def test_fp16x3_product_exact_on_small_integers_and_catches_dropped_terms():
+    g = orc.rng(92)
+    A = g.integers(-1024, 1025, size=(9, 40)).astype(np.float32)
+    B = g.integers(-1024, 1025, size=(40, 11)).astype(np.float32)
+    assert np.array_equal(orc.fp16x3_product(A, B), A.astype(np.float64) @ B.astype(np.float64))
+    # On general data the emulation meets the 2^-19 per-product bound, and the
+    # hi*hi term alone does not (the bound would catch a dropped product).
+    A = orc.uniform(g, (32, 512)); B = orc.uniform(g, (512, 24))
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    assert np.all(np.abs(orc.fp16x3_product(A, B) - exact) <= 2.0 ** -19 * scale)
+    ah, _, ea = orc.fp16_split_rows(A)
+    bh, _, eb = orc.fp16_split_rows(B.T)
+    hi_only = np.ldexp(ah @ bh.T, ea[:, None] + eb[None, :])
+    assert not np.all(np.abs(hi_only - exact) <= 2.0 ** -19 * scale)

@@ -1,15 +1,26 @@
-// Split-BF16 SGEMM on gfx950 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulate.
+// Split SGEMM on gfx950 MFMA (v_mfma_f32_16x16x32_bf16 / _f16), FP32 accumulate.
 //
 // Splitting: every FP32 operand x is written as
-//   x = h0 + h1 + h2 + r,  h0 = bf16(x), h1 = bf16(x - h0), h2 = bf16(x - h0 - h1)
-// (round-to-nearest-even; the differences are exact in FP32).  NP planes are
-// kept and the products
+//   x = h0 + h1 + h2 + r,  h0 = E(x), h1 = E(x - h0), h2 = E(x - h0 - h1)
+// in a 16-bit element format E (round-to-nearest-even; the differences are
+// exact in FP32).  NP planes are kept and the products
 //   NP = 1 (BF16X1): h0*h0
 //   NP = 2 (BF16X3): h0*h0 + h0*h1 + h1*h0                  |error| <~ 3*2^-16 |a||b|
 //   NP = 3 (BF16X6): + h1*h1 + h0*h2 + h2*h0                |error| <~ 6*2^-24 |a||b|
 // are issued as MFMAs into one FP32 accumulator, smallest terms first.
-// Products of bf16 values are exact in FP32, so the remaining error is the
+// Products of 16-bit values are exact in FP32, so the remaining error is the
 // dropped terms plus FP32 accumulation.
+//
+// E = bf16 (8 significant bits) has FP32's exponent range and needs no
+// scaling.  E = fp16 (11 significant bits: FP16X3, the NP = 2 products, error
+// <~ 3*2^-22 |a||b|) has a narrow exponent range, so every row of op(A) and
+// every column of op(B) is first scaled by a power of two 2^-e, e =
+// exponent(max |x| of that row) - 15, bringing its largest magnitude into
+// [2^14, 2^15) (exact: only the exponent changes); the epilogue multiplies
+// C(m, n) by 2^(e_m + e_n) with ldexp, which is exact unless the true result
+// itself over- or underflows.  Elements more than 2^28 below their row's
+// maximum lose low-order bits to fp16 subnormals; their products are at most
+// 2^-28 of the row's largest, so the effect on C is below 2^-40 |a||b|.
 //
 // Two kernels per call:
 //  1. split_prep (HBM-bound): reads op(A) / op(B) once in any layout and
@@ -17,7 +28,7 @@
 //     contiguous -- zero-padded to whole tiles, so that the core needs no
 //     bounds checks and no layout variants.  Transposed sources go through
 //     an LDS tile so both the read and the write are coalesced.
-//  2. bf16_nt_core (MFMA-bound): one workgroup per BM x BN tile of C, 4 waves
+//  2. split_nt_core (MFMA-bound): one workgroup per BM x BN tile of C, 4 waves
 //     (2 x 2), each wave a (BM/2) x (BN/2) grid of 16x16 fragments whose
 //     FP32 accumulators live in AGPRs; per K step of BK, all NP planes of the
 //     A and B tiles are staged global -> registers -> LDS (two stages, the
@@ -36,14 +47,45 @@ namespace ugemm {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Element formats of the planes.
+struct FmtBf16 {
+    typedef __bf16 E;
+    typedef bf16x8 V8;
+    static constexpr bool SCALED = false;
+    static constexpr const char *name = "bf16";
+    __device__ static __forceinline__ floatx4 mfma(const V8 &a, const V8 &b, const floatx4 &c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+struct FmtF16 {
+    typedef _Float16 E;
+    typedef f16x8 V8;
+    static constexpr bool SCALED = true;
+    static constexpr const char *name = "fp16";
+    __device__ static __forceinline__ floatx4 mfma(const V8 &a, const V8 &b, const floatx4 &c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Scale exponent of a row from the bit pattern of its largest magnitude:
+// x * 2^-row_exp lies in (-2^15, 2^15).  0, inf and NaN give -15 (the
+// products are then 0 or non-finite whatever the scale).
+__device__ __forceinline__ int row_exp(unsigned amax_bits) {
+    int e = 0;
+    const float a = __uint_as_float(amax_bits);
+    if (a != 0.0f && __builtin_isfinite(a)) frexpf(a, &e);
+    return e - 15;
+}
 
 // ---------------------------------------------------------------------------
 // Splitting.
-template <int NP, int W>
-__device__ __forceinline__ void split_vec(const float (&x)[W], __bf16 (&out)[NP][W]) {
+template <int NP, int W, class E>
+__device__ __forceinline__ void split_vec(const float (&x)[W], E (&out)[NP][W]) {
     typedef float fv __attribute__((ext_vector_type(W)));
-    typedef __bf16 bv __attribute__((ext_vector_type(W)));
+    typedef E bv __attribute__((ext_vector_type(W)));
     fv v;
 #pragma unroll
     for (int i = 0; i < W; ++i) v[i] = x[i];
@@ -68,16 +110,16 @@ __device__ __forceinline__ void split_vec(const float (&x)[W], __bf16 (&out)[NP]
     }
 }
 
-template <int NP>
-__device__ __forceinline__ void write_planes(__bf16 *dst, long plane_stride, const float (&x)[8]) {
-    __bf16 o[NP][8];
+template <int NP, class FMT>
+__device__ __forceinline__ void write_planes(typename FMT::E *dst, long plane_stride, const float (&x)[8]) {
+    typename FMT::E o[NP][8];
     split_vec<NP, 8>(x, o);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        bf16x8 w;
+        typename FMT::V8 w;
 #pragma unroll
         for (int j = 0; j < 8; ++j) w[j] = o[p][j];
-        *reinterpret_cast<bf16x8 *>(dst + p * plane_stride) = w;
+        *reinterpret_cast<typename FMT::V8 *>(dst + p * plane_stride) = w;
     }
 }
 
@@ -94,10 +136,12 @@ __device__ __forceinline__ long blocked_offset(int r, int k, int rows_pad) {
 
 // Source element (r, k) of the operand: KCONT ? X[r*ld + k] : X[k*ld + r];
 // zeros outside rows x K.  KCONT version: one thread per 8 k of one row.
-template <int NP>
+// FMT::SCALED: row r is multiplied by 2^-row_exp(amax[r]) before splitting.
+template <int NP, class FMT>
 __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict__ X, long ld, int rows, int K,
-                                                        bool vec, __bf16 *__restrict__ out, long plane_stride,
-                                                        int rows_pad, int k_pad) {
+                                                        bool vec, typename FMT::E *__restrict__ out,
+                                                        long plane_stride, int rows_pad, int k_pad,
+                                                        const unsigned *__restrict__ amax) {
     const long chunks_per_row = k_pad / 8;
     const long total = (long)rows_pad * chunks_per_row;
     for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x) {
@@ -113,15 +157,21 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
 #pragma unroll
             for (int j = 0; j < 8; ++j) x[j] = (r < rows && k + j < K) ? p[j] : 0.0f;
         }
-        write_planes<NP>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
+        if constexpr (FMT::SCALED) {
+            const int e = -row_exp(amax[r]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
+        }
+        write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
     }
 }
 
 // Row-contiguous source (X[k*ld + r]): 64 x 64 tiles transposed through LDS.
-template <int NP>
+template <int NP, class FMT>
 __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict__ X, long ld, int rows, int K,
-                                                        bool vec, __bf16 *__restrict__ out, long plane_stride,
-                                                        int rows_pad, int k_pad) {
+                                                        bool vec, typename FMT::E *__restrict__ out,
+                                                        long plane_stride, int rows_pad, int k_pad,
+                                                        const unsigned *__restrict__ amax) {
     __shared__ float tile[64][65];
     const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
     const int t = threadIdx.x;
@@ -154,8 +204,79 @@ __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict_
         float x[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = tile[kk + j][rr];
-        write_planes<NP>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
+        if constexpr (FMT::SCALED) {
+            const int e = -row_exp(amax[r]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
+        }
+        write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
     }
+}
+
+// Largest magnitude of every plane row (row r of op(A), column r of op(B))
+// as IEEE bits, combined with atomicMax on the bit pattern (non-negative
+// floats order like their bit patterns).  amax must be zeroed first.
+__global__ void fill_zero_u32(unsigned *__restrict__ p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// k-contiguous source: block b covers row b / kchunks, k in [2048 * (b % kchunks), +2048).
+__global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                      bool vec, long kchunks, unsigned *__restrict__ amax) {
+    const long b = blockIdx.x;
+    const int r = (int)(b / kchunks);
+    const int k = (int)(b % kchunks) * 2048 + threadIdx.x * 8;
+    const float *p = X + (long)r * ld + k;
+    float m = 0.0f;
+    if (k + 8 <= K && vec) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        const float4 c = *reinterpret_cast<const float4 *>(p + 4);
+        m = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))),
+                  fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(fabsf(c.z), fabsf(c.w))));
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k + j < K) m = fmaxf(m, fabsf(p[j]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        if (r < rows && m > 0.0f) atomicMax(&amax[r], __float_as_uint(m));
+    }
+}
+
+// Row-contiguous source (X[k*ld + r]): block b covers columns r in
+// [256 * (b / kblocks), +256) and k in [256 * (b % kblocks), +256).
+__global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                      bool vec, long kblocks, unsigned *__restrict__ amax) {
+    const long b = blockIdx.x;
+    const int c0 = (int)(b / kblocks) * 256, k0 = (int)(b % kblocks) * 256;
+    const int cg = threadIdx.x & 63, kp = threadIdx.x >> 6;
+    const int c = c0 + cg * 4;
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kk = kp; kk < 256 && k0 + kk < K; kk += 4) {
+        const float *p = X + (long)(k0 + kk) * ld + c;
+        if (vec && c + 4 <= rows) {
+            const float4 a = *reinterpret_cast<const float4 *>(p);
+            m[0] = fmaxf(m[0], fabsf(a.x)); m[1] = fmaxf(m[1], fabsf(a.y));
+            m[2] = fmaxf(m[2], fabsf(a.z)); m[3] = fmaxf(m[3], fabsf(a.w));
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (c + j < rows) m[j] = fmaxf(m[j], fabsf(p[j]));
+        }
+    }
+    __shared__ float sm[4][256];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sm[kp][cg * 4 + j] = m[j];
+    __syncthreads();
+    const int t = threadIdx.x, col = c0 + t;
+    const float v = fmaxf(fmaxf(sm[0][t], sm[1][t]), fmaxf(sm[2][t], sm[3][t]));
+    if (col < rows && v > 0.0f) atomicMax(&amax[col], __float_as_uint(v));
 }
 
 // ---------------------------------------------------------------------------
@@ -189,8 +310,9 @@ struct Products<3> {
     static constexpr int a[6] = {2, 0, 1, 1, 0, 0}, b[6] = {0, 2, 1, 0, 1, 0};
 };
 
-template <int BM_, int BN_, int BK_, int NP_>
+template <int BM_, int BN_, int BK_, int NP_, class FMT_ = FmtBf16>
 struct CoreCfg {
+    using FMT = FMT_;
     static constexpr int BM = BM_, BN = BN_, BK = BK_, NP = NP_;
     static constexpr int WGM = 2, WGN = 2, NT = 256;
     static constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -206,15 +328,22 @@ struct CoreCfg {
     static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 32 == 0, "tile shape");
 };
 
+// amax_a / amax_b: per-row largest magnitudes of the scaled formats
+// (FMT::SCALED), unused otherwise.
 template <class CF>
-__global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, const __bf16 *__restrict__ Ap,
-                                                       long a_plane, int Mp, const __bf16 *__restrict__ Bp,
+__global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, const typename CF::FMT::E *__restrict__ Ap,
+                                                       long a_plane, int Mp, const typename CF::FMT::E *__restrict__ Bp,
                                                        long b_plane, int Np, float alpha, float beta,
-                                                       float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+                                                       float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
+                                                       const unsigned *__restrict__ amax_a,
+                                                       const unsigned *__restrict__ amax_b) {
+    using FMT = typename CF::FMT;
+    using E = typename FMT::E;
+    using V8 = typename FMT::V8;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
     constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
-    __bf16 *lds = reinterpret_cast<__bf16 *>(smem_raw);
+    E *lds = reinterpret_cast<E *>(smem_raw);
 
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
@@ -230,26 +359,26 @@ __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, con
     constexpr int A_CH = NP * BM * CPR / CF::NT;         // 16-byte chunks per thread from A
     static_assert((NP * BM * CPR) % CF::NT == 0 && (BM * CPR) % CF::NT == 0 && (BN * CPR) % CF::NT == 0,
                   "stage split");
-    const __bf16 *a_src = Ap + (long)m0 * BK + t * 8;
-    const __bf16 *b_src = Bp + (long)n0 * BK + t * 8;
-    bf16x8 stage_regs[PER_T];
+    const E *a_src = Ap + (long)m0 * BK + t * 8;
+    const E *b_src = Bp + (long)n0 * BK + t * 8;
+    V8 stage_regs[PER_T];
     auto load_stage = [&](int kt) {
         const long a_k = (long)kt * Mp * BK, b_k = (long)kt * Np * BK;
 #pragma unroll
         for (int i = 0; i < PER_T; ++i) {
             if (i < A_CH) {
                 const int e = i * CF::NT * 8, p = e / (BM * BK), off = e % (BM * BK);
-                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(a_src + p * a_plane + a_k + off);
+                stage_regs[i] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + a_k + off);
             } else {
                 const int e = (i - A_CH) * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
-                stage_regs[i] = *reinterpret_cast<const bf16x8 *>(b_src + p * b_plane + b_k + off);
+                stage_regs[i] = *reinterpret_cast<const V8 *>(b_src + p * b_plane + b_k + off);
             }
         }
     };
     auto store_stage = [&](int buf) {
-        __bf16 *base = lds + buf * STAGE + t * 8;
+        E *base = lds + buf * STAGE + t * 8;
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<bf16x8 *>(base + i * CF::NT * 8) = stage_regs[i];
+        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<V8 *>(base + i * CF::NT * 8) = stage_regs[i];
     };
 
     floatx4 acc[FM][FN];
@@ -268,23 +397,23 @@ __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, con
         const int buf = (STAGES == 2) ? (kt & 1) : 0;
         const bool more = kt + 1 < ktiles;
         if (more) load_stage(kt + 1);
-        const __bf16 *as = lds + buf * STAGE;
-        const __bf16 *bs = as + NP * CF::A_PLANE;
+        const E *as = lds + buf * STAGE;
+        const E *bs = as + NP * CF::A_PLANE;
 #pragma unroll
         for (int s = 0; s < CF::KSTEPS; ++s) {
-            bf16x8 fa[NP][FM], fb[NP][FN];
+            V8 fa[NP][FM], fb[NP][FN];
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
 #pragma unroll
                 for (int i = 0; i < FM; ++i) {
                     const int row = wm * CF::WTM + i * 16 + frow;
-                    fa[p][i] = *reinterpret_cast<const bf16x8 *>(as + p * CF::A_PLANE + row * BK +
+                    fa[p][i] = *reinterpret_cast<const V8 *>(as + p * CF::A_PLANE + row * BK +
                                                                  swz_chunk<CPR>(row, s * 4 + fc) * 8);
                 }
 #pragma unroll
                 for (int j = 0; j < FN; ++j) {
                     const int row = wn * CF::WTN + j * 16 + frow;
-                    fb[p][j] = *reinterpret_cast<const bf16x8 *>(bs + p * CF::B_PLANE + row * BK +
+                    fb[p][j] = *reinterpret_cast<const V8 *>(bs + p * CF::B_PLANE + row * BK +
                                                                  swz_chunk<CPR>(row, s * 4 + fc) * 8);
                 }
             }
@@ -294,28 +423,32 @@ __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, con
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
                     for (int j = 0; j < FN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            fa[Products<NP>::a[q]][i], fb[Products<NP>::b[q]][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = FMT::mfma(fa[Products<NP>::a[q]][i], fb[Products<NP>::b[q]][j], acc[i][j]);
         }
         if (STAGES == 1) __syncthreads();
         if (more) store_stage(STAGES == 2 ? (buf ^ 1) : 0);
         __syncthreads();
     }
 
-    // Epilogue: C = alpha * acc (+ beta * C), rows 4*(l/16)+r, column l%16 of each fragment.
+    // Epilogue: C = alpha * acc (+ beta * C), rows 4*(l/16)+r, column l%16 of each fragment;
+    // scaled formats first undo the row and column scales: acc * 2^(e_m + e_n).
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+            int en = 0;
+            if constexpr (FMT::SCALED) en = row_exp(amax_b[n]);   // n < Np: inside the array
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4 + r;
                 if (interior || (m < M && n < N)) {
+                    float a = acc[i][j][r];
+                    if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
                     float *cp = C + (long)m * ldc + n;
-                    const float v = alpha * acc[i][j][r];
-                    *cp = (beta == 0.0f) ? v : fmaf(alpha, acc[i][j][r], beta * *cp);
+                    const float v = alpha * a;
+                    *cp = (beta == 0.0f) ? v : fmaf(alpha, a, beta * *cp);
                 }
             }
         }
@@ -323,13 +456,8 @@ __global__ __launch_bounds__(CF::NT) void bf16_nt_core(int M, int N, int Kp, con
 }
 
 // ---------------------------------------------------------------------------
-// Variants: (tile, BK) per plane count.
-struct VariantInfo {
-    const char *name;
-    int bm, bn, bk;
-};
-
-template <int NP>
+// Variants: (tile, BK) per form.
+template <int NP, class FMT = FmtBf16>
 struct Table;
 // NP = 1: 256x256x32 (32 KiB/stage); NP = 2: 256x256x32 (64 KiB/stage);
 // NP = 3: 256x128x32 (72 KiB/stage).  Smaller tiles serve small grids.
@@ -354,6 +482,13 @@ struct Table<3> {
     using V2 = CoreCfg<128, 256, 32, 3>;
     using V3 = CoreCfg<64, 64, 32, 3>;
 };
+template <>
+struct Table<2, FmtF16> {
+    using V0 = CoreCfg<256, 256, 32, 2, FmtF16>;
+    using V1 = CoreCfg<128, 128, 32, 2, FmtF16>;
+    using V2 = CoreCfg<256, 128, 32, 2, FmtF16>;
+    using V3 = CoreCfg<64, 64, 32, 2, FmtF16>;
+};
 constexpr int kNumVariants = 4;
 
 template <class CF>
@@ -361,8 +496,8 @@ const char *cfg_name() {
     // Formatted once per instantiation; function-local static initialisation is thread-safe.
     static const std::string name = [] {
         char buf[48];
-        snprintf(buf, sizeof buf, "bf16x%d_nt_%dx%dx%d", CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN,
-                 CF::BK);
+        snprintf(buf, sizeof buf, "%sx%d_nt_%dx%dx%d", CF::FMT::name, CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6),
+                 CF::BM, CF::BN, CF::BK);
         return std::string(buf);
     }();
     return name.c_str();
@@ -373,47 +508,77 @@ int planes_of(int mode) {
         case UGEMM_MODE_BF16X1: return 1;
         case UGEMM_MODE_BF16X3: return 2;
         case UGEMM_MODE_BF16X6: return 3;
+        case UGEMM_MODE_FP16X3: return 2;
         default: return 0;
     }
 }
 
 static long round_up(long x, long a) { return (x + a - 1) / a * a; }
 
-template <int NP>
-int prep_operand(const float *X, long ld, bool kcont, int rows, int K, __bf16 *out, long plane_stride, int rows_pad,
-                 int k_pad, hipStream_t s) {
+// Scaled formats: amax (rows_pad entries, zeroed by the caller) receives the
+// per-row largest magnitudes before the split reads them.
+template <int NP, class FMT>
+int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename FMT::E *out, long plane_stride,
+                 int rows_pad, int k_pad, unsigned *amax, hipStream_t s) {
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
+    if constexpr (FMT::SCALED) {
+        if (rows > 0 && K > 0) {
+            if (kcont) {
+                const long kchunks = (K + 2047) / 2048;
+                hipLaunchKernelGGL(row_amax_kcont, dim3((unsigned)(rows * kchunks)), dim3(256), 0, s, X, ld, rows, K,
+                                   vec, kchunks, amax);
+            } else {
+                const long kblocks = (K + 255) / 256, cblocks = (rows + 255) / 256;
+                hipLaunchKernelGGL(row_amax_rcont, dim3((unsigned)(cblocks * kblocks)), dim3(256), 0, s, X, ld, rows,
+                                   K, vec, kblocks, amax);
+            }
+        }
+    }
     if (kcont) {
         const long total = (long)rows_pad * (k_pad / 8);
         long blocks = (total + 255) / 256;
         if (blocks > 8192) blocks = 8192;
-        hipLaunchKernelGGL(split_prep_kcont<NP>, dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, vec, out,
-                           plane_stride, rows_pad, k_pad);
+        hipLaunchKernelGGL((split_prep_kcont<NP, FMT>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, vec,
+                           out, plane_stride, rows_pad, k_pad, (const unsigned *)amax);
     } else {
         dim3 grid((unsigned)((rows_pad + 63) / 64), (unsigned)((k_pad + 63) / 64));
-        hipLaunchKernelGGL(split_prep_rcont<NP>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, plane_stride,
-                           rows_pad, k_pad);
+        hipLaunchKernelGGL((split_prep_rcont<NP, FMT>), grid, dim3(256), 0, s, X, ld, rows, K, vec, out, plane_stride,
+                           rows_pad, k_pad, (const unsigned *)amax);
     }
     UGEMM_HIP_TRY(hipGetLastError());
     return UGEMM_OK;
 }
 
+// Workspace: NP planes of A (Mp x Kp) and of B (Np x Kp), then (scaled
+// formats) Mp + Np row maxima.
 template <class CF>
 int run_cfg(const Gemm<float> &g, hipStream_t s) {
+    using FMT = typename CF::FMT;
+    using E = typename FMT::E;
     constexpr int NP = CF::NP;
     const long Mp = round_up(g.M, CF::BM), Np = round_up(g.N, CF::BN), Kp = round_up(g.K, 64);
     const long a_plane = Mp * Kp, b_plane = Np * Kp;
-    const size_t bytes = (size_t)NP * (a_plane + b_plane) * sizeof(__bf16);
+    const size_t plane_bytes = (size_t)NP * (a_plane + b_plane) * sizeof(E);
+    const size_t amax_bytes = FMT::SCALED ? (size_t)(Mp + Np) * sizeof(unsigned) : 0;
     const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
     if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
+    if (FMT::SCALED && (long)g.M * ((g.K + 2047) / 2048) > 0x7fffffffL)
+        return fail(UGEMM_ERR_ARG, "too many row blocks");
     void *ws = nullptr;
-    int rc = stream_alloc(&ws, bytes, s);
+    int rc = stream_alloc(&ws, plane_bytes + amax_bytes, s);
     if (rc) return rc;
-    __bf16 *Ap = (__bf16 *)ws, *Bp = Ap + NP * a_plane;
-    rc = prep_operand<NP>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, s);
-    if (!rc) rc = prep_operand<NP>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, s);
+    E *Ap = (E *)ws, *Bp = Ap + NP * a_plane;
+    unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + plane_bytes) : nullptr;
+    unsigned *amax_b = FMT::SCALED ? amax_a + Mp : nullptr;
+    if (FMT::SCALED) {
+        // A fill kernel rather than a memset: callers may capture this call in a graph.
+        hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
+                           s, amax_a, Mp + Np);
+    }
+    rc = prep_operand<NP, FMT>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s);
+    if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s);
     if (!rc) {
-        auto kern = bf16_nt_core<CF>;
+        auto kern = split_nt_core<CF>;
         // Once per kernel instantiation (thread-safe static initialisation).
         static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
@@ -421,31 +586,32 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
         if (!rc) {
             timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
-                               (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n);
+                               (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m,
+                               tiles_n, (const unsigned *)amax_a, (const unsigned *)amax_b);
             timing_end(s);
             hipError_t e = hipGetLastError();
-            if (e != hipSuccess) rc = hip_fail(e, "bf16_nt_core launch");
+            if (e != hipSuccess) rc = hip_fail(e, "split core launch");
         }
     }
     int rc2 = stream_free(ws, s);
     return rc ? rc : rc2;
 }
 
-template <int NP>
+template <int NP, class FMT = FmtBf16>
 int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
-    using T = Table<NP>;
+    using T = Table<NP, FMT>;
     switch (variant) {
         case 0: return run_cfg<typename T::V0>(g, s);
         case 1: return run_cfg<typename T::V1>(g, s);
         case 2: return run_cfg<typename T::V2>(g, s);
         case 3: return run_cfg<typename T::V3>(g, s);
-        default: return fail(UGEMM_ERR_ARG, "split-bf16: no variant %d", variant);
+        default: return fail(UGEMM_ERR_ARG, "split form: no variant %d", variant);
     }
 }
 
-template <int NP>
+template <int NP, class FMT = FmtBf16>
 const char *name_np(int v) {
-    using T = Table<NP>;
+    using T = Table<NP, FMT>;
     switch (v) {
         case 0: return cfg_name<typename T::V0>();
         case 1: return cfg_name<typename T::V1>();
@@ -461,10 +627,11 @@ int mfma_variant_count(int mode) { return planes_of(mode) ? kNumVariants : 0; }
 
 const char *mfma_variant_name(int mode, int v) {
     if (v < 0 || v >= kNumVariants) return nullptr;
-    switch (planes_of(mode)) {
-        case 1: return name_np<1>(v);
-        case 2: return name_np<2>(v);
-        case 3: return name_np<3>(v);
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: return name_np<1>(v);
+        case UGEMM_MODE_BF16X3: return name_np<2>(v);
+        case UGEMM_MODE_BF16X6: return name_np<3>(v);
+        case UGEMM_MODE_FP16X3: return name_np<2, FmtF16>(v);
         default: return nullptr;
     }
 }
@@ -482,11 +649,12 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
 }
 
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
-    switch (planes_of(mode)) {
-        case 1: return launch_np<1>(g, variant, s);
-        case 2: return launch_np<2>(g, variant, s);
-        case 3: return launch_np<3>(g, variant, s);
-        default: return fail(UGEMM_ERR_UNSUPPORTED, "mode %d is not a split-bf16 form", mode);
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: return launch_np<1>(g, variant, s);
+        case UGEMM_MODE_BF16X3: return launch_np<2>(g, variant, s);
+        case UGEMM_MODE_BF16X6: return launch_np<3>(g, variant, s);
+        case UGEMM_MODE_FP16X3: return launch_np<2, FmtF16>(g, variant, s);
+        default: return fail(UGEMM_ERR_UNSUPPORTED, "mode %d is not a split form", mode);
     }
 }
 

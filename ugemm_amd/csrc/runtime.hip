@@ -247,7 +247,8 @@ int run_sgemm(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
         case UGEMM_MODE_FP32: return launch_fp32(g, variant, s);
         case UGEMM_MODE_BF16X1:
         case UGEMM_MODE_BF16X3:
-        case UGEMM_MODE_BF16X6: return launch_mfma(g, mode, variant, s);
+        case UGEMM_MODE_BF16X6:
+        case UGEMM_MODE_FP16X3: return launch_mfma(g, mode, variant, s);
         case UGEMM_MODE_FP8_BLOCK: return launch_fp8(g, variant, s);
         default: return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
     }
@@ -334,6 +335,7 @@ static int env_mode() {
             if (!strcmp(e, "bf16x1")) return (int)UGEMM_MODE_BF16X1;
             if (!strcmp(e, "bf16x6")) return (int)UGEMM_MODE_BF16X6;
             if (!strcmp(e, "fp8")) return (int)UGEMM_MODE_FP8_BLOCK;
+            if (!strcmp(e, "fp16x3")) return (int)UGEMM_MODE_FP16X3;
         }
         return (int)UGEMM_MODE_FP32;
     }();
@@ -468,7 +470,8 @@ int ugemm_hip_variant_count(int mode) {
         case UGEMM_MODE_FP32: return fp32_variant_count();
         case UGEMM_MODE_FP8_BLOCK: return fp8_variant_count();
         case -1: return dgemm_variant_count();
-        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: return mfma_variant_count(mode);
+        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: case UGEMM_MODE_FP16X3:
+            return mfma_variant_count(mode);
         default: return 0;
     }
 }
@@ -478,7 +481,8 @@ const char *ugemm_hip_variant_name(int mode, int v) {
         case UGEMM_MODE_FP32: return fp32_variant_name(v);
         case UGEMM_MODE_FP8_BLOCK: return fp8_variant_name(v);
         case -1: return dgemm_variant_name(v);
-        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: return mfma_variant_name(mode, v);
+        case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: case UGEMM_MODE_FP16X3:
+            return mfma_variant_name(mode, v);
         default: return nullptr;
     }
 }

@@ -30,7 +30,9 @@ struct Entry {
 using Key = std::tuple<int, int, int, int, int, int>;
 std::mutex g_mu;
 std::map<Key, Entry> g_table;
-thread_local int t_forced[UGEMM_MODE_COUNT + 1] = {-1, -1, -1, -1, -1, -1};
+// Forced variant per mode (slot UGEMM_MODE_COUNT: DGEMM); stored + 1 so that
+// zero-initialisation means "none".
+thread_local int t_forced_p1[UGEMM_MODE_COUNT + 1] = {};
 
 int lg(int x) {
     int b = 0;
@@ -71,7 +73,7 @@ int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K) {
     return it == g_table.end() ? -1 : it->second.variant;
 }
 
-int forced_variant(int mode) { return t_forced[slot(mode)]; }
+int forced_variant(int mode) { return t_forced_p1[slot(mode)] - 1; }
 
 void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -87,7 +89,7 @@ extern "C" {
 int ugemm_hip_force_variant(int mode, int variant) {
     if (mode < -1 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
     if (variant >= ugemm_hip_variant_count(mode)) return fail(UGEMM_ERR_ARG, "mode %d has no variant %d", mode, variant);
-    t_forced[slot(mode)] = variant < 0 ? -1 : variant;
+    t_forced_p1[slot(mode)] = variant < 0 ? 0 : variant + 1;
     return UGEMM_OK;
 }
 
