@@ -41,7 +41,7 @@ def test_split_layouts_shapes(mode, major, ta, tb):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("variant", range(6))
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_split_every_variant(mode, variant, ta, tb):
     ug.force_variant(mode, variant)

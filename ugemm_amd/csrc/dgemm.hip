@@ -105,18 +105,30 @@ __global__ __launch_bounds__(CF::NT) void dgemm_mfma_kernel(int M, int N, int K,
         __syncthreads();
     }
 
+    // C values of one fragment row are loaded before any is stored (a load
+    // cannot pass a possibly aliasing store: one latency per row, not per element).
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4);
+        double cv[FN][4];
+        if (beta != 0.0) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[j][r] = (interior || (mb + 4 * r < M && n < N)) ? C[(long)(mb + 4 * r) * ldc + n] : 0.0;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) + 4 * r;
-                if (interior || (m < M && n < N)) {
-                    double *cp = C + (long)m * ldc + n;
-                    *cp = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * *cp);
-                }
+                const int m = mb + 4 * r;
+                if (interior || (m < M && n < N))
+                    C[(long)m * ldc + n] = (beta == 0.0) ? alpha * acc[i][j][r]
+                                                         : fma(alpha, acc[i][j][r], beta * cv[j][r]);
             }
         }
     }

@@ -39,6 +39,7 @@
 // l%16.  A and B use the same k assignment, so any order of the hardware's
 // internal sum pairs the right elements.
 #include <string>
+#include <utility>
 
 #include "internal.h"
 #include "kernel_common.h"
@@ -310,11 +311,41 @@ struct Products<3> {
     static constexpr int a[6] = {2, 0, 1, 1, 0, 0}, b[6] = {0, 2, 1, 0, 1, 0};
 };
 
-template <int BM_, int BN_, int BK_, int NP_, class FMT_ = FmtBf16>
+// First product (in Products order) that uses plane p of A / of B: the
+// phased core reads a plane's fragments from LDS right before that product.
+template <int NP>
+constexpr bool first_use_a(int q) {
+    for (int r = 0; r < q; ++r)
+        if (Products<NP>::a[r] == Products<NP>::a[q]) return false;
+    return true;
+}
+template <int NP>
+constexpr bool first_use_b(int q) {
+    for (int r = 0; r < q; ++r)
+        if (Products<NP>::b[r] == Products<NP>::b[q]) return false;
+    return true;
+}
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// WGM x WGN waves per workgroup.  PHASED: fragments of each plane are read
+// just before the first product that needs them, behind a scheduling
+// barrier, so that at most three planes' fragments are live (2 waves per
+// SIMD fit in 256 registers); the other wave on the SIMD covers the LDS
+// latency.  Otherwise all fragments of a K step are read first.
+template <int BM_, int BN_, int BK_, int NP_, class FMT_ = FmtBf16, int WGM_ = 2, int WGN_ = 2, bool PHASED_ = false>
 struct CoreCfg {
     using FMT = FMT_;
     static constexpr int BM = BM_, BN = BN_, BK = BK_, NP = NP_;
-    static constexpr int WGM = 2, WGN = 2, NT = 256;
+    static constexpr bool PHASED = PHASED_;
+    static constexpr int WGM = WGM_, WGN = WGN_, NT = 64 * WGM_ * WGN_;
     static constexpr int WTM = BM / WGM, WTN = BN / WGN;
     static constexpr int FM = WTM / 16, FN = WTN / 16;
     static constexpr int KSTEPS = BK / 32, CPR = BK / 8;
@@ -402,6 +433,35 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
 #pragma unroll
         for (int s = 0; s < CF::KSTEPS; ++s) {
             V8 fa[NP][FM], fb[NP][FN];
+            if constexpr (CF::PHASED) {
+                static_for<Products<NP>::N>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int pa = Products<NP>::a[q], pb = Products<NP>::b[q];
+                    if constexpr (q > 0 && (first_use_a<NP>(q) || first_use_b<NP>(q)))
+                        __builtin_amdgcn_sched_barrier(0);
+                    if constexpr (first_use_a<NP>(q)) {
+#pragma unroll
+                        for (int i = 0; i < FM; ++i) {
+                            const int row = wm * CF::WTM + i * 16 + frow;
+                            fa[pa][i] = *reinterpret_cast<const V8 *>(as + pa * CF::A_PLANE + row * BK +
+                                                                      swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                        }
+                    }
+                    if constexpr (first_use_b<NP>(q)) {
+#pragma unroll
+                        for (int j = 0; j < FN; ++j) {
+                            const int row = wn * CF::WTN + j * 16 + frow;
+                            fb[pb][j] = *reinterpret_cast<const V8 *>(bs + pb * CF::B_PLANE + row * BK +
+                                                                      swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[pa][i], fb[pb][j], acc[i][j]);
+                });
+                continue;
+            }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
 #pragma unroll
@@ -432,23 +492,44 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
 
     // Epilogue: C = alpha * acc (+ beta * C), rows 4*(l/16)+r, column l%16 of each fragment;
     // scaled formats first undo the row and column scales: acc * 2^(e_m + e_n).
+    // The C values of JB fragments are all loaded before any is stored: a
+    // load cannot move above a store to C that might alias it, so
+    // interleaving them would expose one memory latency per element.  With
+    // 256 accumulators per lane there is no room for the batch (it spills):
+    // those variants interleave.
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+    constexpr bool BATCH = FM * FN * 4 < 256;
+    constexpr int JB = !BATCH ? 1 : (CF::NT >= 1024 ? 1 : FN);
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
+    for (int j0 = 0; j0 < FN; j0 += JB) {
+        const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
+        float cv[JB][4];
+        if (BATCH && beta != 0.0f) {
+#pragma unroll
+            for (int jj = 0; jj < JB; ++jj) {
+                const int n = n0 + wn * CF::WTN + (j0 + jj) * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[jj][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj) {
+            const int j = j0 + jj;
             const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
             int en = 0;
             if constexpr (FMT::SCALED) en = row_exp(amax_b[n]);   // n < Np: inside the array
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4 + r;
+                const int m = mb + r;
                 if (interior || (m < M && n < N)) {
                     float a = acc[i][j][r];
                     if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
                     float *cp = C + (long)m * ldc + n;
-                    const float v = alpha * a;
-                    *cp = (beta == 0.0f) ? v : fmaf(alpha, a, beta * *cp);
+                    if (beta == 0.0f) *cp = alpha * a;
+                    else *cp = fmaf(alpha, a, beta * (BATCH ? cv[jj][r] : *cp));
                 }
             }
         }
@@ -459,14 +540,19 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
 // Variants: (tile, BK) per form.
 template <int NP, class FMT = FmtBf16>
 struct Table;
-// NP = 1: 256x256x32 (32 KiB/stage); NP = 2: 256x256x32 (64 KiB/stage);
-// NP = 3: 256x128x32 (72 KiB/stage).  Smaller tiles serve small grids.
+// V0-V3: 4 waves (2 x 2), one per SIMD.  V4, V5: 8 or 16 waves, 2 or 4 per
+// SIMD, which hide each other's LDS and global latency (measured at 8192^3:
+// 15-20% faster core than V0).  NP = 1, 2: 256x256x32 tiles (32 / 64 KiB
+// per stage); NP = 3: 256x128x32 (72 KiB per stage).  Smaller tiles serve
+// small grids.
 template <>
 struct Table<1> {
     using V0 = CoreCfg<256, 256, 32, 1>;
     using V1 = CoreCfg<128, 128, 32, 1>;
     using V2 = CoreCfg<256, 128, 32, 1>;
     using V3 = CoreCfg<64, 64, 32, 1>;
+    using V4 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, false>;
+    using V5 = CoreCfg<256, 256, 32, 1, FmtBf16, 4, 4, true>;
 };
 template <>
 struct Table<2> {
@@ -474,6 +560,8 @@ struct Table<2> {
     using V1 = CoreCfg<128, 128, 32, 2>;
     using V2 = CoreCfg<256, 128, 32, 2>;
     using V3 = CoreCfg<64, 64, 32, 2>;
+    using V4 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, false>;
+    using V5 = CoreCfg<256, 256, 32, 2, FmtBf16, 4, 4, true>;
 };
 template <>
 struct Table<3> {
@@ -481,6 +569,8 @@ struct Table<3> {
     using V1 = CoreCfg<128, 128, 32, 3>;
     using V2 = CoreCfg<128, 256, 32, 3>;
     using V3 = CoreCfg<64, 64, 32, 3>;
+    using V4 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true>;
+    using V5 = CoreCfg<256, 128, 32, 3, FmtBf16, 4, 2, true>;
 };
 template <>
 struct Table<2, FmtF16> {
@@ -488,16 +578,19 @@ struct Table<2, FmtF16> {
     using V1 = CoreCfg<128, 128, 32, 2, FmtF16>;
     using V2 = CoreCfg<256, 128, 32, 2, FmtF16>;
     using V3 = CoreCfg<64, 64, 32, 2, FmtF16>;
+    using V4 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, false>;
+    using V5 = CoreCfg<256, 256, 32, 2, FmtF16, 4, 4, true>;
 };
-constexpr int kNumVariants = 4;
+constexpr int kNumVariants = 6;
 
 template <class CF>
 const char *cfg_name() {
     // Formatted once per instantiation; function-local static initialisation is thread-safe.
     static const std::string name = [] {
         char buf[48];
-        snprintf(buf, sizeof buf, "%sx%d_nt_%dx%dx%d", CF::FMT::name, CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6),
-                 CF::BM, CF::BN, CF::BK);
+        snprintf(buf, sizeof buf, "%sx%d_nt_%dx%dx%d_w%dx%d%s", CF::FMT::name,
+                 CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN, CF::BK, CF::WGM, CF::WGN,
+                 CF::PHASED ? "_ph" : "");
         return std::string(buf);
     }();
     return name.c_str();
@@ -605,6 +698,8 @@ int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
         case 1: return run_cfg<typename T::V1>(g, s);
         case 2: return run_cfg<typename T::V2>(g, s);
         case 3: return run_cfg<typename T::V3>(g, s);
+        case 4: return run_cfg<typename T::V4>(g, s);
+        case 5: return run_cfg<typename T::V5>(g, s);
         default: return fail(UGEMM_ERR_ARG, "split form: no variant %d", variant);
     }
 }
@@ -617,6 +712,8 @@ const char *name_np(int v) {
         case 1: return cfg_name<typename T::V1>();
         case 2: return cfg_name<typename T::V2>();
         case 3: return cfg_name<typename T::V3>();
+        case 4: return cfg_name<typename T::V4>();
+        case 5: return cfg_name<typename T::V5>();
         default: return nullptr;
     }
 }
@@ -639,10 +736,11 @@ const char *mfma_variant_name(int mode, int v) {
 int mfma_pick_variant(const Gemm<float> &g, int mode) {
     const long cus = device_cu_count();
     const int np = planes_of(mode);
-    // Largest tile whose grid still gives every CU work.
+    // Largest tile whose grid still gives every CU work; there, the
+    // multi-wave variant measured fastest for the form.
     const long big_m = 256, big_n = (np == 3) ? 128 : 256;
     const long tiles_big = ((g.M + big_m - 1) / big_m) * ((g.N + big_n - 1) / big_n);
-    if (tiles_big >= cus) return 0;
+    if (tiles_big >= cus) return 5;
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= cus / 2) return 1;
     return 3;

@@ -215,18 +215,34 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
         __syncthreads();
     }
 
+    // C values of one fragment row are loaded before any is stored (a load
+    // cannot pass a possibly aliasing store: one latency per row, not per
+    // element) -- unless 256 accumulators per lane leave no room (it spills).
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+    constexpr bool BATCH = FM * FN * 4 < 256;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
+        float cv[FN][4];
+        if (BATCH && beta != 0.0f) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    cv[j][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4 + r;
+                const int m = mb + r;
                 if (interior || (m < M && n < N)) {
                     float *cp = C + (long)m * ldc + n;
-                    *cp = (beta == 0.0f) ? alpha * acc[i][j][r] : fmaf(alpha, acc[i][j][r], beta * *cp);
+                    if (beta == 0.0f) *cp = alpha * acc[i][j][r];
+                    else *cp = fmaf(alpha, acc[i][j][r], beta * (BATCH ? cv[j][r] : *cp));
                 }
             }
         }
