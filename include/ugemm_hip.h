@@ -95,6 +95,37 @@ int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int 
                         double alpha, const double *dA, int lda, const double *dB, int ldb,
                         double beta, double *dC, int ldc, void *stream);
 
+/* Bias + activation epilogue, applied after alpha/beta in this order:
+ *   C[m][n] += bias[m] (UGEMM_BIAS_ROWS) or bias[n] (UGEMM_BIAS_COLS);
+ *   C = C > 0 ? C : slope * C (UGEMM_ACT_LEAKY_RELU)
+ * -- the post-GEMM loop of gl_convolution_LReLU (sgemm_gl1.h:210-217), fused
+ * into the GEMM's store.  m, n index the caller's C (either major).  Modes:
+ * FP32 and the split forms (FP8 returns UGEMM_ERR_UNSUPPORTED).  d_bias may
+ * be NULL (activation only). */
+enum { UGEMM_BIAS_ROWS = 0, UGEMM_BIAS_COLS = 1 };
+enum { UGEMM_ACT_NONE = 0, UGEMM_ACT_LEAKY_RELU = 1 };
+int ugemm_hip_sgemm_ex_dev(char major, char transa, char transb, int M, int N, int K,
+                           float alpha, const float *dA, int lda, const float *dB, int ldb,
+                           float beta, float *dC, int ldc, int mode,
+                           const float *d_bias, int bias_dim, int act, float slope, void *stream);
+
+/* Convolution as one implicit GEMM (the column matrix of im2col,
+ * sgemm_gl1.h:166-190, is gathered inside the GEMM's tile loader and never
+ * stored), with the bias + LeakyReLU epilogue:
+ *   outputs[c][p] = act(sum_q weights[c][q] * col[q][p] + bias[c])
+ *   inputs [ich][h][w], weights [ch][ich*k*k], outputs [ch][hcol*wcol],
+ *   hcol = (h + 2*pad - k)/stride + 1, wcol likewise.
+ * convolution_LReLU_hip has the argument list of gl_convolution_LReLU
+ * (sgemm_gl1.h:191-218) and ocl_convolution_LReLU (sgemm_ocl1.h:302-337):
+ * host pointers, synchronous, slope 0.1; status in ugemm_hip_last_error().
+ * ugemm_hip_conv2d_dev: device pointers, stream-ordered, act/slope/bias
+ * selectable (d_bias may be NULL); mode UGEMM_MODE_FP32. */
+void convolution_LReLU_hip(float *inputs, int ich, int w, int h, float *weights, int k, int pad, int stride,
+                           float *outputs, int ch, float *bias);
+int ugemm_hip_conv2d_dev(const float *d_in, int ich, int w, int h, const float *d_weights, int k, int pad,
+                         int stride, float *d_out, int ch, const float *d_bias, int act, float slope, int mode,
+                         void *stream);
+
 /* ------------------------------------------------------------------ */
 /* 4. Grouped (batched, uneven) SGEMM: one persistent launch.          */
 /*    Every problem is row-major; pointers are device pointers.        */

@@ -23,6 +23,11 @@
  *       (gemm_cpu.h:9-11) and BLAS alpha == 0 semantics (row-major indexing,
  *       where gemm_cpu.h:308-323 always indexes column-major).
  *
+ *   oracle_im2col / oracle_conv_lrelu
+ *       im2col (sgemm_gl1.h:166-190) and the convolution + bias + LeakyReLU
+ *       of gl_convolution_LReLU (sgemm_gl1.h:191-218) in double precision,
+ *       with the elementwise error scale (end of this file).
+ *
  * Pins: tests/test_oracle.py checks these functions against the known-answer
  * product of sgemm_test.c:186-200, exact small-integer products, numpy's FP64
  * matmul, brute force on tiny shapes, and the golden vectors produced by the
@@ -298,3 +303,72 @@ void NAME(char major, char ta, char tb, int M, int N, int K, T alpha, const T *A
 
 DEFINE_PACKED(float, oracle_sgemm_packed)
 DEFINE_PACKED(double, oracle_dgemm_packed)
+
+/* ------------------------------------------------------------------------
+ * Convolution (the reference's CNN path).
+ *
+ * oracle_im2col restates im2col of sgemm_gl1.h:166-190: channels_col =
+ * channels * kernel_h * kernel_w rows; row c takes image channel
+ * c / kernel_h / kernel_w at offset (h_offset, w_offset) = ((c / kernel_w) %
+ * kernel_h, c % kernel_w); entries outside the image are 0.
+ *
+ * oracle_conv_lrelu restates gl_convolution_LReLU (sgemm_gl1.h:191-218) in
+ * double precision: im2col, the row-major NN product weights (ch x K) times
+ * col (K x N), then out += bias[i] and out = out > 0 ? out : slope * out
+ * (act != 0; the reference's slope is 0.1).  scale (may be NULL) receives
+ * sum_q |w[i][q]| |col[q][p]| + |bias[i]|, the elementwise error scale of a
+ * single-precision evaluation.  Returns 0, or -1 when a buffer cannot be
+ * allocated.
+ * ------------------------------------------------------------------------ */
+void oracle_im2col(const float *im, int channels, int height, int width, int kernel_h, int kernel_w, int pad_h,
+                   int pad_w, int stride_h, int stride_w, float *col)
+{
+	int height_col = (height + 2 * pad_h - kernel_h) / stride_h + 1;
+	int width_col = (width + 2 * pad_w - kernel_w) / stride_w + 1;
+	int channels_col = channels * kernel_h * kernel_w;
+	for (int c = 0; c < channels_col; c++) {
+		int w_offset = c % kernel_w;
+		int h_offset = (c / kernel_w) % kernel_h;
+		int c_im = c / kernel_h / kernel_w;
+		for (int h = 0; h < height_col; h++) {
+			for (int w = 0; w < width_col; w++) {
+				int h_pad = h * stride_h - pad_h + h_offset;
+				int w_pad = w * stride_w - pad_w + w_offset;
+				long o = ((long)c * height_col + h) * width_col + w;
+				if (h_pad >= 0 && h_pad < height && w_pad >= 0 && w_pad < width)
+					col[o] = im[((long)c_im * height + h_pad) * width + w_pad];
+				else
+					col[o] = 0.0f;
+			}
+		}
+	}
+}
+
+int oracle_conv_lrelu(const float *inputs, int ich, int w, int h, const float *weights, int k, int pad, int stride,
+                      double *outputs, int ch, const float *bias, int act, double slope, double *scale)
+{
+	int hcol = (h + 2 * pad - k) / stride + 1, wcol = (w + 2 * pad - k) / stride + 1;
+	long K = (long)ich * k * k, N = (long)hcol * wcol;
+	float *col = (float *)malloc(sizeof(float) * (size_t)(K * N > 0 ? K * N : 1));
+	if (!col) return -1;
+	oracle_im2col(inputs, ich, h, w, k, k, pad, pad, stride, stride, col);
+	for (int i = 0; i < ch; i++) {
+		for (long p = 0; p < N; p++) {
+			double s = 0.0, a = 0.0;
+			for (long q = 0; q < K; q++) {
+				double t = (double)weights[(long)i * K + q] * (double)col[q * N + p];
+				s += t;
+				a += fabs(t);
+			}
+			if (bias) {
+				s += (double)bias[i];
+				a += fabs((double)bias[i]);
+			}
+			if (act && s <= 0.0) s *= slope;
+			outputs[(long)i * N + p] = s;
+			if (scale) scale[(long)i * N + p] = a;
+		}
+	}
+	free(col);
+	return 0;
+}

@@ -51,6 +51,10 @@ def lib():
             fn = getattr(L, name)
             fn.restype = i
             fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp, i, vp, vp, i]
+        L.oracle_im2col.restype = None
+        L.oracle_im2col.argtypes = [vp, i, i, i, i, i, i, i, i, i, vp]
+        L.oracle_conv_lrelu.restype = i
+        L.oracle_conv_lrelu.argtypes = [vp, i, i, i, vp, i, i, i, vp, i, vp, i, d, vp]
         _o = L
     return _o
 
@@ -317,3 +321,35 @@ def fp16x3_product(Aop, Bop):
     bh, bl, eb = fp16_split_rows(np.asarray(Bop).T)
     P = ah @ bh.T + ah @ bl.T + al @ bh.T
     return np.ldexp(P, ea[:, None] + eb[None, :])
+
+
+# ---------------------------------------------------------------------------
+# Convolution (sgemm_gl1.h:166-218): bindings to oracle.c.
+
+def conv_out_shape(h, w, k, pad, stride):
+    return (h + 2 * pad - k) // stride + 1, (w + 2 * pad - k) // stride + 1
+
+
+def im2col(im, ich, h, w, k, pad, stride):
+    """(ich*k*k, hcol*wcol) column matrix of a CHW float32 image."""
+    hcol, wcol = conv_out_shape(h, w, k, pad, stride)
+    im = np.ascontiguousarray(im, np.float32)
+    col = np.zeros((ich * k * k, hcol * wcol), np.float32)
+    lib().oracle_im2col(_p(im), ich, h, w, k, k, pad, pad, stride, stride, _p(col))
+    return col
+
+
+def conv_lrelu(inputs, ich, w, h, weights, k, pad, stride, ch, bias=None, act=1, slope=0.1):
+    """(out, scale): float64 [ch, hcol*wcol] of gl_convolution_LReLU's result
+    and its elementwise error scale."""
+    hcol, wcol = conv_out_shape(h, w, k, pad, stride)
+    inputs = np.ascontiguousarray(inputs, np.float32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    b = None if bias is None else np.ascontiguousarray(bias, np.float32)
+    out = np.zeros((ch, hcol * wcol), np.float64)
+    scale = np.zeros((ch, hcol * wcol), np.float64)
+    rc = lib().oracle_conv_lrelu(_p(inputs), ich, w, h, _p(weights), k, pad, stride, _p(out), ch,
+                                 None if b is None else _p(b), act, slope, _p(scale))
+    if rc:
+        raise MemoryError("oracle_conv_lrelu")
+    return out, scale

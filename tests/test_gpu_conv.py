@@ -1,0 +1,168 @@
+"""Fused bias + LeakyReLU epilogue and implicit-GEMM convolution vs the oracle.
+
+The convolution oracle restates gl_convolution_LReLU (sgemm_gl1.h:191-218):
+im2col (sgemm_gl1.h:166-190), the weights x col product, out += bias[c],
+out = out > 0 ? out : 0.1 * out.  Acceptance: the FP32 elementwise bound
+2(K+1) eps32 * scale with scale = sum |w||col| + |bias| (the activation only
+shrinks errors).  The GEMM epilogue is checked against the oracle's exact
+product followed by the same bias/activation."""
+import numpy as np
+import pytest
+
+import ugemm_amd as ug
+from ugemm_amd import hip
+from oracle import oracle as orc
+from tests.helpers import Case, MODE_EXTRA
+
+pytestmark = pytest.mark.gpu
+
+CONV_SHAPES = [
+    # ich, h, w, k, pad, stride, ch
+    (1, 5, 5, 3, 1, 1, 1),
+    (3, 17, 13, 3, 1, 1, 16),
+    (16, 14, 14, 3, 1, 1, 32),
+    (8, 28, 28, 3, 1, 2, 24),
+    (4, 9, 11, 5, 2, 1, 7),
+    (32, 7, 7, 1, 0, 1, 64),
+    (3, 33, 31, 4, 2, 2, 5),
+    (64, 14, 14, 3, 1, 1, 130),
+]
+
+
+def _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, act, slope, variant=None):
+    hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
+    d_x, d_w = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(wts)
+    d_b = hip.DeviceArray.from_host(bias) if bias is not None else None
+    d_o = hip.DeviceArray.from_host(np.full((ch, hcol * wcol), np.nan, np.float32))
+    if variant is not None:
+        ug.force_variant(ug.MODE_FP32, variant)
+    try:
+        ug.conv2d_dev(d_x, ich, w, h, d_w, k, pad, stride, d_o, ch, bias=d_b, act=act, slope=slope)
+    finally:
+        if variant is not None:
+            ug.force_variant(ug.MODE_FP32, -1)
+    hip.synchronize()
+    return d_o.download()
+
+
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+@pytest.mark.parametrize("act,with_bias", [(ug.ACT_LEAKY_RELU, True), (ug.ACT_NONE, False)])
+def test_conv2d_matches_oracle(shape, act, with_bias):
+    ich, h, w, k, pad, stride, ch = shape
+    g = orc.rng(hash(shape) % (1 << 31))
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,)) if with_bias else None
+    print("conv", shape, act, with_bias, flush=True)
+    got = _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, act, 0.1)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=act, slope=0.1)
+    ok, worst = orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))
+    assert ok, f"{shape}: worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("variant", range(4))
+def test_conv2d_every_tile_variant(variant):
+    ich, h, w, k, pad, stride, ch = 6, 15, 12, 3, 1, 1, 70
+    g = orc.rng(700 + variant)
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,))
+    got = _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, ug.ACT_LEAKY_RELU, 0.1, variant=variant)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias)
+    assert orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))[0]
+
+
+def test_convolution_LReLU_host_entry():
+    """The gl_convolution_LReLU argument list, host arrays, slope 0.1."""
+    ich, h, w, k, pad, stride, ch = 3, 20, 18, 3, 1, 1, 12
+    g = orc.rng(801)
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,))
+    hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
+    out = np.zeros((ch, hcol * wcol), np.float32)
+    ug.convolution_LReLU(x, ich, w, h, wts, k, pad, stride, out, ch, bias)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias)
+    assert orc.check_bound(out, exact, scale, orc.tolerance(ich * k * k))[0]
+    assert np.any(out < 0) and np.any(out > 0)      # both branches of the activation
+
+
+def test_conv2d_argument_errors():
+    L = ug.load()
+    d = hip.DeviceArray(64, np.float32)
+    assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 5, 0, 1, d.ptr, 1, None, 1, 0.1, 0, None) == ug.ERR_ARG
+    assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 0, d.ptr, 1, None, 1, 0.1, 0, None) == ug.ERR_ARG
+    assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 7, 0.1, 0, None) == ug.ERR_ARG
+    assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 1, 0.1,
+                                  ug.MODE_BF16X3, None) == ug.ERR_UNSUPPORTED
+
+
+# ---------------------------------------------------------------------------
+# GEMM with the fused epilogue.
+
+def _epilogue(v, bias, bias_dim, act, slope, major):
+    """The epilogue applied to a logical [m, n] result (either major)."""
+    out = v.copy()
+    if bias is not None:
+        b = bias.astype(np.float64)
+        out += b[:, None] if bias_dim == ug.BIAS_ROWS else b[None, :]
+    if act == ug.ACT_LEAKY_RELU:
+        out = np.where(out > 0, out, slope * out)
+    return out
+
+
+@pytest.mark.parametrize("mode", [ug.MODE_FP32, ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X6])
+@pytest.mark.parametrize("major,ta,tb", [("R", "N", "N"), ("R", "T", "N"), ("C", "N", "T"), ("C", "T", "T")])
+@pytest.mark.parametrize("bias_dim", [ug.BIAS_ROWS, ug.BIAS_COLS])
+def test_sgemm_epilogue(mode, major, ta, tb, bias_dim):
+    M, N, K = 150, 97, 133
+    case = Case(major, ta, tb, M, N, K, 1.5, 0.5, seed=900 + mode * 10 + bias_dim, pad=1)
+    g = orc.rng(950 + bias_dim)
+    bias = orc.uniform(g, (M if bias_dim == ug.BIAS_ROWS else N,))
+    dA, dB, dC = (hip.DeviceArray.from_host(case.A), hip.DeviceArray.from_host(case.B),
+                  hip.DeviceArray.from_host(case.C0))
+    d_b = hip.DeviceArray.from_host(bias)
+    slope = 0.25
+    ug.sgemm_ex_dev(major, ta, tb, M, N, K, case.alpha, dA, case.lda, dB, case.ldb, case.beta, dC, case.ldc,
+                    mode=mode, bias=d_b, bias_dim=bias_dim, act=ug.ACT_LEAKY_RELU, slope=slope)
+    hip.synchronize()
+    got = case.logical(dC.download())
+    exact, scale = case.exact()
+    want = _epilogue(exact, bias, bias_dim, ug.ACT_LEAKY_RELU, slope, major)
+    bscale = np.abs(bias.astype(np.float64))
+    scale = scale + (bscale[:, None] if bias_dim == ug.BIAS_ROWS else bscale[None, :])
+    ok, worst = orc.check_bound(got, want, scale, orc.tolerance(K, extra=MODE_EXTRA.get(mode, 0.0)))
+    assert ok, f"mode {mode} {major}{ta}{tb} bias_dim {bias_dim}: {worst:.3g}"
+    assert case.untouched_padding(dC.download())
+
+
+def test_sgemm_epilogue_alpha_zero_and_activation_only():
+    """alpha == 0 runs the scaling kernel, which applies the epilogue too;
+    a NULL bias applies the activation alone."""
+    M, N, K = 40, 30, 20
+    case = Case("R", "N", "N", M, N, K, 0.0, 0.5, seed=990)
+    bias = orc.uniform(orc.rng(991), (M,))
+    dA, dB, dC = (hip.DeviceArray.from_host(case.A), hip.DeviceArray.from_host(case.B),
+                  hip.DeviceArray.from_host(case.C0))
+    ug.sgemm_ex_dev("R", "N", "N", M, N, K, 0.0, dA, K, dB, N, 0.5, dC, N, bias=hip.DeviceArray.from_host(bias),
+                    act=ug.ACT_LEAKY_RELU, slope=0.1)
+    hip.synchronize()
+    want = _epilogue(0.5 * case.C0.astype(np.float64), bias, ug.BIAS_ROWS, ug.ACT_LEAKY_RELU, 0.1, "R")
+    assert np.allclose(dC.download(), want, rtol=1e-6, atol=1e-7)
+    dC.upload(case.C0)
+    ug.sgemm_ex_dev("R", "N", "N", M, N, K, 1.0, dA, K, dB, N, 0.0, dC, N, act=ug.ACT_LEAKY_RELU, slope=0.1)
+    hip.synchronize()
+    exact = case.A.astype(np.float64) @ case.B.astype(np.float64)
+    scale = np.abs(case.A.astype(np.float64)) @ np.abs(case.B.astype(np.float64))
+    assert orc.check_bound(dC.download(), np.where(exact > 0, exact, 0.1 * exact), scale, orc.tolerance(K))[0]
+
+
+def test_sgemm_epilogue_rejected_for_fp8():
+    d = hip.DeviceArray(64, np.float32)
+    L = ug.load()
+    rc = L.ugemm_hip_sgemm_ex_dev(b"R", b"N", b"N", 4, 4, 4, 1.0, d.ptr, 4, d.ptr, 4, 0.0, d.ptr, 4,
+                                  ug.MODE_FP8_BLOCK, d.ptr, 0, 1, 0.1, None)
+    assert rc == ug.ERR_UNSUPPORTED
+    rc = L.ugemm_hip_sgemm_ex_dev(b"R", b"N", b"N", 4, 4, 4, 1.0, d.ptr, 4, d.ptr, 4, 0.0, d.ptr, 4,
+                                  ug.MODE_FP32, d.ptr, 5, 1, 0.1, None)
+    assert rc == ug.ERR_ARG

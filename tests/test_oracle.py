@@ -310,3 +310,58 @@ def test_fp16x3_product_exact_on_small_integers_and_catches_dropped_terms():
     bh, _, eb = orc.fp16_split_rows(B.T)
     hi_only = np.ldexp(ah @ bh.T, ea[:, None] + eb[None, :])
     assert not np.all(np.abs(hi_only - exact) <= 2.0 ** -19 * scale)
+
+
+# ---------------------------------------------------------------------------
+# Convolution oracle (im2col + GEMM + bias + LeakyReLU, sgemm_gl1.h:166-218).
+
+def _direct_conv(x, wts, k, pad, stride, bias, act, slope):
+    """Direct convolution from the definition (no im2col): an independent
+    formula for the same result."""
+    ich, h, w = x.shape
+    ch = wts.shape[0]
+    hcol, wcol = (h + 2 * pad - k) // stride + 1, (w + 2 * pad - k) // stride + 1
+    xp = np.zeros((ich, h + 2 * pad, w + 2 * pad))
+    xp[:, pad:pad + h, pad:pad + w] = x
+    wk = wts.reshape(ch, ich, k, k).astype(np.float64)
+    out = np.zeros((ch, hcol, wcol))
+    for i in range(k):
+        for j in range(k):
+            win = xp[:, i:i + stride * hcol:stride, j:j + stride * wcol:stride]     # ich x hcol x wcol
+            out += np.einsum("oc,cyx->oyx", wk[:, :, i, j], win)
+    out = out.reshape(ch, -1)
+    if bias is not None:
+        out += bias.astype(np.float64)[:, None]
+    if act:
+        out = np.where(out > 0, out, slope * out)
+    return out
+
+
+@pytest.mark.parametrize("ich,h,w,k,pad,stride,ch", [(1, 5, 5, 3, 1, 1, 1), (3, 7, 6, 3, 1, 2, 4),
+                                                      (2, 9, 9, 5, 2, 1, 3), (4, 8, 5, 1, 0, 1, 2),
+                                                      (2, 6, 7, 4, 2, 2, 5)])
+def test_conv_oracle_matches_direct_convolution(ich, h, w, k, pad, stride, ch):
+    g = orc.rng(h * 100 + w * 10 + k)
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,))
+    out, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=1, slope=0.1)
+    ref = _direct_conv(x.astype(np.float64), wts, k, pad, stride, bias, 1, 0.1)
+    assert np.allclose(out, ref, rtol=0, atol=1e-12)
+    assert np.all(scale >= np.abs(out) - 1e-12)
+    # planted mistakes: a wrong pad and a wrong activation slope are caught
+    bad = _direct_conv(x.astype(np.float64), wts, k, pad + 1, stride, bias, 1, 0.1) if k > 1 else None
+    if bad is not None and bad.shape == ref.shape:
+        assert not np.allclose(out, bad, atol=1e-6)
+    assert not np.allclose(out, _direct_conv(x.astype(np.float64), wts, k, pad, stride, bias, 1, 0.2), atol=1e-6)
+
+
+def test_im2col_known_answer():
+    # 1 channel 3x3 image, 2x2 kernel, no pad, stride 1: 4 rows (kernel
+    # offsets) x 4 columns (output pixels).
+    x = np.arange(1, 10, dtype=np.float32).reshape(1, 3, 3)
+    col = orc.im2col(x, 1, 3, 3, 2, 0, 1)
+    assert col.tolist() == [[1, 2, 4, 5], [2, 3, 5, 6], [4, 5, 7, 8], [5, 6, 8, 9]]
+    # pad 1, stride 2, 3x3 kernel: the corner taps of the first pixel are outside the image
+    col = orc.im2col(x, 1, 3, 3, 3, 1, 2)
+    assert col.shape == (9, 4) and col[0, 0] == 0 and col[4, 0] == 1 and col[8, 3] == 0 and col[4, 3] == 9

@@ -66,6 +66,9 @@ def load(path=None):
         "sgemm_hip_packed": (None, [c, c, i, i, i, f, vp, vp, f, vp]),
         "ugemm_hip_sgemm_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp]),
         "ugemm_hip_dgemm_dev": (i, [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
+        "ugemm_hip_sgemm_ex_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp, i, i, f, vp]),
+        "ugemm_hip_conv2d_dev": (i, [vp, i, i, i, vp, i, i, i, vp, i, vp, i, f, i, vp]),
+        "convolution_LReLU_hip": (None, [vp, i, i, i, vp, i, i, i, vp, i, vp]),
         "ugemm_hip_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(Problem), i, i, vp]),
         "ugemm_hip_grouped_run": (i, [vp, vp]),
         "ugemm_hip_grouped_plan_destroy": (None, [vp]),
@@ -152,6 +155,35 @@ def dgemm_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC,
     rc = load().ugemm_hip_dgemm_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
                                     _ptr(dB), ldb, beta, _ptr(dC), ldc, s)
     return _raise_if(rc)
+
+
+BIAS_ROWS, BIAS_COLS = 0, 1
+ACT_NONE, ACT_LEAKY_RELU = 0, 1
+
+
+def sgemm_ex_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, mode=MODE_FP32,
+                 bias=None, bias_dim=BIAS_ROWS, act=ACT_NONE, slope=0.1, stream=None):
+    """sgemm_dev followed by the fused bias + activation epilogue."""
+    s = stream.handle if hasattr(stream, "handle") else stream
+    rc = load().ugemm_hip_sgemm_ex_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda, _ptr(dB),
+                                       ldb, beta, _ptr(dC), ldc, mode, _ptr(bias), bias_dim, act, slope, s)
+    return _raise_if(rc)
+
+
+def conv2d_dev(d_in, ich, w, h, d_weights, k, pad, stride, d_out, ch, bias=None, act=ACT_LEAKY_RELU, slope=0.1,
+               mode=MODE_FP32, stream=None):
+    """Implicit-GEMM convolution + bias + activation on device buffers."""
+    s = stream.handle if hasattr(stream, "handle") else stream
+    rc = load().ugemm_hip_conv2d_dev(_ptr(d_in), ich, w, h, _ptr(d_weights), k, pad, stride, _ptr(d_out), ch,
+                                     _ptr(bias), act, slope, mode, s)
+    return _raise_if(rc)
+
+
+def convolution_LReLU(inputs, ich, w, h, weights, k, pad, stride, outputs, ch, bias):
+    """Host-array convolution with the argument list of gl_convolution_LReLU."""
+    L = load()
+    L.convolution_LReLU_hip(_ptr(inputs), ich, w, h, _ptr(weights), k, pad, stride, _ptr(outputs), ch, _ptr(bias))
+    return _raise_if(L.ugemm_hip_last_error())
 
 
 def timing_enable(on=True):

@@ -13,6 +13,7 @@
 // C = alpha * acc + beta * C, C is not read when beta == 0 (sgemm_ocl2.h:86-88).
 #pragma once
 
+#include "internal.h"
 #include "kernel_common.h"
 
 namespace ugemm {
@@ -46,12 +47,17 @@ struct Fp32Cfg {
 // aligned == false (with VEC == 4) reads A and B with scalar loads: the
 // grouped kernel mixes problems whose operands allow 128-bit loads with
 // problems whose operands do not.
-template <class CF, bool TA, bool TB, int VEC>
+// CONV: opB is the implicit im2col matrix of cd (B and TB unused): every
+// thread keeps one column n of the B tile, so the pixel's origin is computed
+// once per tile and each element costs one table read and two compares.
+// epi: bias and activation after alpha/beta (Epilogue, internal.h).
+template <class CF, bool TA, bool TB, int VEC, bool CONV = false>
 __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K, const float alpha,
                                           const float *__restrict__ A, const long lda,
                                           const float *__restrict__ B, const long ldb, const float beta,
                                           float *__restrict__ C, const long ldc, const int m0, const int n0,
-                                          float *lds, const bool aligned = true) {
+                                          float *lds, const bool aligned = true, const Epilogue &epi = Epilogue(),
+                                          const ConvDesc &cd = ConvDesc()) {
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, TM = CF::TM, TN = CF::TN, NT = CF::NT;
     constexpr int LDA = CF::LDA, LDB = CF::LDB;
     float *As = lds;                    // [2][BK][LDA]
@@ -73,22 +79,54 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
     const int ktiles = (K + BK - 1) / BK;
     const int kfull = K / BK;   // tiles with no K tail
 
+    // Implicit im2col: thread t always holds column nl = t % BN of the B
+    // tile (NT is a multiple of BN) and rows kl = t / BN + i * (NT / BN).
+    static_assert(!CONV || (NT % BN == 0 && !TB), "implicit operand layout");
+    constexpr int CONV_N = CONV ? BK * BN / NT : 1;
+    float cvals[CONV_N];
+    long c_base = 0;
+    int c_h0 = 0, c_w0 = 0;
+    bool c_ncol = false;
+    if constexpr (CONV) {
+        const int n = n0 + t % BN;
+        c_ncol = n < N;
+        const int ho = c_ncol ? n / cd.wcol : 0, wo = c_ncol ? n - ho * cd.wcol : 0;
+        c_h0 = ho * cd.stride - cd.pad;
+        c_w0 = wo * cd.stride - cd.pad;
+        c_base = (long)c_h0 * cd.W + c_w0;
+    }
+
     auto load_tile = [&](int kt) {
         const int k0 = kt * BK;
         const float *a0 = TA ? A + (long)k0 * lda + m0 : A + (long)m0 * lda + k0;
-        const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
-        if (interior && kt < kfull && aligned) {
-            ra.template load<false>(a0, lda, t, 0, 0);
-            rb.template load<false>(b0, ldb, t, 0, 0);
+        if (interior && kt < kfull && aligned) ra.template load<false>(a0, lda, t, 0, 0);
+        else ra.template load<true>(a0, lda, t, TA ? M - m0 : K - k0, TA ? K - k0 : M - m0, aligned);
+        if constexpr (CONV) {
+#pragma unroll
+            for (int i = 0; i < CONV_N; ++i) {
+                const int k = k0 + t / BN + i * (NT / BN);
+                float v = 0.0f;
+                if (c_ncol && k < K) {
+                    const int2 e = cd.ktab[k];
+                    const int hi = c_h0 + (e.y >> 16), wi = c_w0 + (e.y & 0xffff);
+                    if ((unsigned)hi < (unsigned)cd.H && (unsigned)wi < (unsigned)cd.W) v = cd.in[c_base + e.x];
+                }
+                cvals[i] = v;
+            }
         } else {
-            const int mr = M - m0, nr = N - n0, kr = K - k0;
-            ra.template load<true>(a0, lda, t, TA ? mr : kr, TA ? kr : mr, aligned);
-            rb.template load<true>(b0, ldb, t, TB ? kr : nr, TB ? nr : kr, aligned);
+            const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
+            if (interior && kt < kfull && aligned) rb.template load<false>(b0, ldb, t, 0, 0);
+            else rb.template load<true>(b0, ldb, t, TB ? K - k0 : N - n0, TB ? N - n0 : K - k0, aligned);
         }
     };
     auto store_tile = [&](int buf) {
         ra.template store<TA, LDA>(As + buf * BK * LDA, t);
-        rb.template store<!TB, LDB>(Bs + buf * BK * LDB, t);
+        if constexpr (CONV) {
+#pragma unroll
+            for (int i = 0; i < CONV_N; ++i) Bs[buf * BK * LDB + (t / BN + i * (NT / BN)) * LDB + t % BN] = cvals[i];
+        } else {
+            rb.template store<!TB, LDB>(Bs + buf * BK * LDB, t);
+        }
     };
 
     float acc[TM][TN];
@@ -141,8 +179,9 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
         __syncthreads();
     }
 
-    // Epilogue: C = alpha * acc (+ beta * C).
+    // Epilogue: C = alpha * acc (+ beta * C), then bias and activation.
     const bool vec_c = interior && ((ldc & 3) == 0) && ((((uintptr_t)C) & 15) == 0);
+    const bool has_epi = epi.active();
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
         const int m = m0 + (i / 4) * CF::TY * 4 + ty * 4 + (i % 4);
@@ -160,13 +199,20 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
                     o = make_float4(fmaf(alpha, acc[i][g * 4 + 0], beta * c.x), fmaf(alpha, acc[i][g * 4 + 1], beta * c.y),
                                     fmaf(alpha, acc[i][g * 4 + 2], beta * c.z), fmaf(alpha, acc[i][g * 4 + 3], beta * c.w));
                 }
+                if (has_epi) {
+                    o.x = apply_epilogue(epi, o.x, m, n);
+                    o.y = apply_epilogue(epi, o.y, m, n + 1);
+                    o.z = apply_epilogue(epi, o.z, m, n + 2);
+                    o.w = apply_epilogue(epi, o.w, m, n + 3);
+                }
                 *reinterpret_cast<float4 *>(cp) = o;
             } else if (m < M) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (n + j < N) {
-                        const float r = alpha * acc[i][g * 4 + j];
-                        cp[j] = (beta == 0.0f) ? r : fmaf(alpha, acc[i][g * 4 + j], beta * cp[j]);
+                        float r = (beta == 0.0f) ? alpha * acc[i][g * 4 + j] : fmaf(alpha, acc[i][g * 4 + j], beta * cp[j]);
+                        if (has_epi) r = apply_epilogue(epi, r, m, n + j);
+                        cp[j] = r;
                     }
                 }
             }

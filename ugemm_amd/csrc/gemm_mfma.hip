@@ -367,7 +367,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                                                        long b_plane, int Np, float alpha, float beta,
                                                        float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
                                                        const unsigned *__restrict__ amax_a,
-                                                       const unsigned *__restrict__ amax_b) {
+                                                       const unsigned *__restrict__ amax_b, Epilogue epi) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
@@ -498,6 +498,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     // 256 accumulators per lane there is no room for the batch (it spills):
     // those variants interleave.
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+    const bool has_epi = epi.active();
     constexpr bool BATCH = FM * FN * 4 < 256;
     constexpr int JB = !BATCH ? 1 : (CF::NT >= 1024 ? 1 : FN);
 #pragma unroll
@@ -528,8 +529,9 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                     float a = acc[i][j][r];
                     if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
                     float *cp = C + (long)m * ldc + n;
-                    if (beta == 0.0f) *cp = alpha * a;
-                    else *cp = fmaf(alpha, a, beta * (BATCH ? cv[jj][r] : *cp));
+                    float v = (beta == 0.0f) ? alpha * a : fmaf(alpha, a, beta * (BATCH ? cv[jj][r] : *cp));
+                    if (has_epi) v = apply_epilogue(epi, v, m, n);
+                    *cp = v;
                 }
             }
         }
@@ -680,7 +682,7 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
             timing_begin(s);
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
                                (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m,
-                               tiles_n, (const unsigned *)amax_a, (const unsigned *)amax_b);
+                               tiles_n, (const unsigned *)amax_a, (const unsigned *)amax_b, g.epi);
             timing_end(s);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = hip_fail(e, "split core launch");

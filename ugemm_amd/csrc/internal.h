@@ -7,6 +7,34 @@
 
 namespace ugemm {
 
+// Optional epilogue after C = alpha * AB + beta * C, in this order (the
+// post-GEMM loop of gl_convolution_LReLU, sgemm_gl1.h:210-217):
+//   C[m, n] += bias[rows ? m : n];  C[m, n] = C > 0 ? C : slope * C  (act == 1)
+struct Epilogue {
+    const float *bias = nullptr;
+    int bias_rows = 1;     // bias indexed by the row m (else by the column n)
+    int act = 0;           // 0: none, 1: LeakyReLU
+    float slope = 0.1f;
+    __host__ __device__ bool active() const { return bias != nullptr || act != 0; }
+};
+
+__host__ __device__ inline float apply_epilogue(const Epilogue &e, float v, int m, int n) {
+    if (e.bias) v += e.bias[e.bias_rows ? m : n];
+    if (e.act == 1) v = v > 0.0f ? v : v * e.slope;
+    return v;
+}
+
+// Implicit im2col operand (the column matrix of im2col, sgemm_gl1.h:166-190,
+// never materialised): element (k, n) of the K x N matrix, K = ich*kh*kw,
+// N = hcol*wcol, is in[c*H*W + (ho*stride - pad + i)*W + (wo*stride - pad + j)]
+// for k = (c*kh + i)*kw + j, n = ho*wcol + wo, and 0 outside the image.
+// ktab[k] = {c*H*W + i*W + j, (i << 16) | j}, built on the device per call.
+struct ConvDesc {
+    const float *in;
+    const int2 *ktab;
+    int H, W, wcol, stride, pad;
+};
+
 // One row-major GEMM after major/transpose normalisation:
 //   C[m, n] = alpha * sum_k opA(m, k) * opB(k, n) + beta * C[m, n]
 //   opA(m, k) = ta ? A[k * lda + m] : A[m * lda + k]
@@ -19,6 +47,7 @@ struct Gemm {
     const T *B; long ldb;
     T *C; long ldc;
     bool ta, tb;
+    Epilogue epi{};
 };
 
 // ---- status (runtime.hip) ----
@@ -54,8 +83,12 @@ const char *dgemm_variant_name(int v);
 int dgemm_pick_variant(const Gemm<double> &g);
 int launch_dgemm(const Gemm<double> &g, int variant, hipStream_t s);
 
-// C <- beta * C for the alpha == 0 / K == 0 cases (beta == 0 writes zeros).
-int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s);
+// C <- beta * C for the alpha == 0 / K == 0 cases (beta == 0 writes zeros),
+// followed by the epilogue when one is given.
+int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s, const Epilogue &epi = Epilogue());
+
+// Implicit-GEMM convolution (FP32 tile kernel): g.B is unused, opB comes from cd.
+int launch_conv_fp32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s);
 int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s);
 
 // ---- tuning (tune.hip) ----

@@ -155,29 +155,36 @@ void timing_end(hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// C <- beta * C kernels (alpha == 0 or K == 0). beta == 0 writes zeros.
+// C <- beta * C kernels (alpha == 0 or K == 0). beta == 0 writes zeros; the
+// FP32 version then applies the optional epilogue.
 template <typename T>
-__global__ void scale_kernel(int M, int N, T beta, T *__restrict__ C, long ldc) {
+__global__ void scale_kernel(int M, int N, T beta, T *__restrict__ C, long ldc, Epilogue epi) {
     const long total = (long)M * N;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const long m = i / N, n = i - m * N;
         T *p = C + m * ldc + n;
-        *p = (beta == T(0)) ? T(0) : beta * *p;
+        T v = (beta == T(0)) ? T(0) : beta * *p;
+        if constexpr (sizeof(T) == 4) v = apply_epilogue(epi, v, (int)m, (int)n);
+        *p = v;
     }
 }
 
 template <typename T>
-static int launch_scale(int M, int N, T beta, T *C, long ldc, hipStream_t s) {
+static int launch_scale(int M, int N, T beta, T *C, long ldc, hipStream_t s, const Epilogue &epi) {
     const long total = (long)M * N;
     const int threads = 256;
     long blocks = (total + threads - 1) / threads;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0, s, M, N, beta, C, ldc);
+    hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0, s, M, N, beta, C, ldc, epi);
     UGEMM_HIP_TRY(hipGetLastError());
     return UGEMM_OK;
 }
-int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s) { return launch_scale(M, N, beta, C, ldc, s); }
-int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s) { return launch_scale(M, N, beta, C, ldc, s); }
+int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s, const Epilogue &epi) {
+    return launch_scale(M, N, beta, C, ldc, s, epi);
+}
+int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s) {
+    return launch_scale(M, N, beta, C, ldc, s, Epilogue());
+}
 
 // ---------------------------------------------------------------------------
 // Argument normalisation: validate, then express every call as a row-major
@@ -239,9 +246,11 @@ static int choose_variant(const Gemm<float> &g, int mode) {
 int run_sgemm(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
     if (g.M == 0 || g.N == 0) return UGEMM_OK;
     if (g.alpha == 0.0f || g.K == 0) {
-        if (g.beta == 1.0f) return UGEMM_OK;
-        return launch_scale_f32(g.M, g.N, g.beta, g.C, g.ldc, s);
+        if (g.beta == 1.0f && !g.epi.active()) return UGEMM_OK;
+        return launch_scale_f32(g.M, g.N, g.beta, g.C, g.ldc, s, g.epi);
     }
+    if (g.epi.active() && (mode == UGEMM_MODE_FP8_BLOCK))
+        return fail(UGEMM_ERR_UNSUPPORTED, "bias/activation epilogue: not provided for the FP8 form");
     if (variant < 0) variant = choose_variant(g, mode);
     switch (mode) {
         case UGEMM_MODE_FP32: return launch_fp32(g, variant, s);
@@ -342,6 +351,52 @@ static int env_mode() {
     return mode;
 }
 
+// Implicit-GEMM convolution: M = ch, N = hcol * wcol, K = ich * k * k.
+static int conv_shape(int ich, int w, int h, int k, int pad, int stride, int ch, int *hcol, int *wcol) {
+    if (ich < 1 || w < 1 || h < 1 || k < 1 || ch < 1 || stride < 1 || pad < 0 || k > 32767)
+        return fail(UGEMM_ERR_ARG, "conv: bad shape ich=%d w=%d h=%d k=%d pad=%d stride=%d ch=%d", ich, w, h, k, pad,
+                    stride, ch);
+    if (h + 2 * pad < k || w + 2 * pad < k) return fail(UGEMM_ERR_ARG, "conv: kernel larger than the padded image");
+    *hcol = (h + 2 * pad - k) / stride + 1;
+    *wcol = (w + 2 * pad - k) / stride + 1;
+    if ((long)ich * k * k > 0x7fffffffL || (long)*hcol * *wcol > 0x7fffffffL || (long)ich * h * w > 0x7fffffffL)
+        return fail(UGEMM_ERR_ARG, "conv: sizes exceed 32-bit indexing");
+    return UGEMM_OK;
+}
+
+__global__ void conv_ktab_kernel(int2 *__restrict__ ktab, int K, int k, int H, int W) {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk >= K) return;
+    const int c = kk / (k * k), r = kk - c * k * k, i = r / k, j = r - i * k;
+    ktab[kk] = make_int2(c * H * W + i * W + j, (i << 16) | j);
+}
+
+static int run_conv(const float *d_in, int ich, int w, int h, const float *d_weights, int k, int pad, int stride,
+                    float *d_out, int ch, const Epilogue &epi, int mode, hipStream_t s) {
+    int hcol, wcol;
+    int rc = conv_shape(ich, w, h, k, pad, stride, ch, &hcol, &wcol);
+    if (rc) return rc;
+    if (mode != UGEMM_MODE_FP32) return fail(UGEMM_ERR_UNSUPPORTED, "conv: mode %d not provided (FP32 only)", mode);
+    if (!d_in || !d_weights || !d_out) return fail(UGEMM_ERR_ARG, "conv: NULL buffer");
+    const int K = ich * k * k;
+    void *ws = nullptr;
+    rc = stream_alloc(&ws, (size_t)K * sizeof(int2), s);
+    if (rc) return rc;
+    int2 *ktab = (int2 *)ws;
+    hipLaunchKernelGGL(conv_ktab_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, ktab, K, k, h, w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "conv: offset table");
+    if (!rc) {
+        Gemm<float> g{ch, hcol * wcol, K, 1.0f, 0.0f, d_weights, K, nullptr, 0, d_out, (long)hcol * wcol, false, false};
+        g.epi = epi;
+        const ConvDesc cd{d_in, ktab, h, w, wcol, stride, pad};
+        int v = forced_variant(UGEMM_MODE_FP32);
+        rc = launch_conv_fp32(g, cd, v, s);
+    }
+    const int rc2 = stream_free(ws, s);
+    return rc ? rc : rc2;
+}
+
 }  // namespace ugemm
 
 using namespace ugemm;
@@ -428,6 +483,80 @@ int ugemm_hip_sgemm_dev(char major, char transa, char transb, int M, int N, int 
     if (rc) return rc;
     rc = run_sgemm(g, mode, -1, (hipStream_t)stream);
     return rc ? rc : ok();
+}
+
+int ugemm_hip_sgemm_ex_dev(char major, char transa, char transb, int M, int N, int K, float alpha,
+                           const float *dA, int lda, const float *dB, int ldb, float beta, float *dC, int ldc,
+                           int mode, const float *d_bias, int bias_dim, int act, float slope, void *stream) {
+    if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    if (bias_dim != UGEMM_BIAS_ROWS && bias_dim != UGEMM_BIAS_COLS) return fail(UGEMM_ERR_ARG, "bad bias_dim %d", bias_dim);
+    if (act != UGEMM_ACT_NONE && act != UGEMM_ACT_LEAKY_RELU) return fail(UGEMM_ERR_ARG, "bad activation %d", act);
+    int rc = ensure_device();
+    if (rc) return rc;
+    Gemm<float> g;
+    rc = normalize(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, g);
+    if (rc) return rc;
+    const bool row = major == 'R' || major == 'r';
+    // Column-major calls run transposed (normalize): a per-row bias of the
+    // caller's C is then a per-column bias of the kernel's C.
+    g.epi.bias = d_bias;
+    g.epi.bias_rows = (bias_dim == UGEMM_BIAS_ROWS) == row ? 1 : 0;
+    g.epi.act = act;
+    g.epi.slope = slope;
+    rc = run_sgemm(g, mode, -1, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+int ugemm_hip_conv2d_dev(const float *d_in, int ich, int w, int h, const float *d_weights, int k, int pad, int stride,
+                         float *d_out, int ch, const float *d_bias, int act, float slope, int mode, void *stream) {
+    if (act != UGEMM_ACT_NONE && act != UGEMM_ACT_LEAKY_RELU) return fail(UGEMM_ERR_ARG, "bad activation %d", act);
+    int rc = ensure_device();
+    if (rc) return rc;
+    Epilogue epi;
+    epi.bias = d_bias;
+    epi.bias_rows = 1;
+    epi.act = act;
+    epi.slope = slope;
+    rc = run_conv(d_in, ich, w, h, d_weights, k, pad, stride, d_out, ch, epi, mode, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+void convolution_LReLU_hip(float *inputs, int ich, int w, int h, float *weights, int k, int pad, int stride,
+                           float *outputs, int ch, float *bias) {
+    // Host pointers, synchronous: the argument list of gl_convolution_LReLU
+    // (sgemm_gl1.h:191) and ocl_convolution_LReLU (sgemm_ocl1.h:302).
+    int rc = ensure_device();
+    int hcol = 0, wcol = 0;
+    if (!rc) rc = conv_shape(ich, w, h, k, pad, stride, ch, &hcol, &wcol);
+    if (!rc && (!inputs || !weights || !outputs || !bias)) rc = fail(UGEMM_ERR_ARG, "conv: NULL buffer");
+    if (rc) return;
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    if (hipSetDevice(g_dev.device) != hipSuccess) {
+        hip_fail(hipGetLastError(), "hipSetDevice");
+        return;
+    }
+    hipStream_t s = g_dev.stream;
+    const size_t n_in = (size_t)ich * h * w, n_w = (size_t)ch * ich * k * k, n_out = (size_t)ch * hcol * wcol;
+    void *ws = nullptr;
+    rc = stream_alloc(&ws, (n_in + n_w + n_out + (size_t)ch) * sizeof(float), s);
+    if (rc) return;
+    float *d_in = (float *)ws, *d_w = d_in + n_in, *d_out = d_w + n_w, *d_b = d_out + n_out;
+    hipError_t e = hipMemcpyAsync(d_in, inputs, n_in * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_w, weights, n_w * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_b, bias, (size_t)ch * sizeof(float), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "conv: copy in");
+    if (!rc) {
+        Epilogue epi;
+        epi.bias = d_b;
+        epi.act = UGEMM_ACT_LEAKY_RELU;
+        epi.slope = 0.1f;
+        rc = run_conv(d_in, ich, w, h, d_w, k, pad, stride, d_out, ch, epi, UGEMM_MODE_FP32, s);
+    }
+    if (!rc && (e = hipMemcpyAsync(outputs, d_out, n_out * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        rc = hip_fail(e, "conv: copy out");
+    const int rc2 = stream_free(ws, s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess && !rc) rc = hip_fail(e, "conv: synchronize");
+    if (!rc && !rc2) ok();
 }
 
 int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int K, double alpha, const double *dA,

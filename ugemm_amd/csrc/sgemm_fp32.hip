@@ -11,11 +11,27 @@ template <class CF, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(CF::NT) void sgemm_fp32_kernel(int M, int N, int K, float alpha,
                                                             const float *__restrict__ A, long lda,
                                                             const float *__restrict__ B, long ldb, float beta,
-                                                            float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+                                                            float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
+                                                            Epilogue epi) {
     __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    fp32_tile<CF, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm * CF::BM, tn * CF::BN, lds);
+    fp32_tile<CF, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm * CF::BM, tn * CF::BN, lds, true, epi);
+}
+
+// Implicit-GEMM convolution: C (M = output channels x N = output pixels) =
+// alpha * W (M x K, row-major weights) * im2col(in) + beta * C, then the
+// epilogue.  The column matrix is gathered in the tile loader (ConvDesc).
+template <class CF, bool TA, int VEC>
+__global__ __launch_bounds__(CF::NT) void conv_fp32_kernel(int M, int N, int K, float alpha,
+                                                           const float *__restrict__ A, long lda, float beta,
+                                                           float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
+                                                           Epilogue epi, ConvDesc cd) {
+    __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    fp32_tile<CF, TA, false, VEC, true>(M, N, K, alpha, A, lda, nullptr, 0, beta, C, ldc, tm * CF::BM, tn * CF::BN,
+                                        lds, true, epi, cd);
 }
 
 // ---------------------------------------------------------------------------
@@ -40,7 +56,7 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     dim3 grid((unsigned)nblk), block(CF::NT);
 #define UGEMM_L(TA, TB, VEC)                                                                             \
     hipLaunchKernelGGL((sgemm_fp32_kernel<CF, TA, TB, VEC>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, \
-                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n)
+                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi)
     timing_begin(s);
     if (vec) {
         if (!g.ta && !g.tb) UGEMM_L(false, false, 4);
@@ -55,6 +71,29 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     }
     timing_end(s);
 #undef UGEMM_L
+    UGEMM_HIP_TRY(hipGetLastError());
+    return UGEMM_OK;
+}
+
+template <class CF>
+int launch_conv_cfg(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
+    const int tiles_m = (g.M + CF::BM - 1) / CF::BM;
+    const int tiles_n = (g.N + CF::BN - 1) / CF::BN;
+    const long nblk = (long)tiles_m * tiles_n;
+    if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "conv: too many tiles (%ld)", nblk);
+    const bool vec = ugemm_aligned16(g.A) && (g.lda % 4 == 0);
+    dim3 grid((unsigned)nblk), block(CF::NT);
+    timing_begin(s);
+    if (vec && !g.ta)
+        hipLaunchKernelGGL((conv_fp32_kernel<CF, false, 4>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+    else if (!g.ta)
+        hipLaunchKernelGGL((conv_fp32_kernel<CF, false, 1>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+    else
+        hipLaunchKernelGGL((conv_fp32_kernel<CF, true, 1>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+    timing_end(s);
     UGEMM_HIP_TRY(hipGetLastError());
     return UGEMM_OK;
 }
@@ -74,6 +113,17 @@ int fp32_pick_variant(const Gemm<float> &g) {
     if (g.M >= 4 * g.N && t64 >= cus) return 2;
     if (g.N >= 4 * g.M && t64 >= cus) return 3;
     return 1;
+}
+
+int launch_conv_fp32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s) {
+    if (variant < 0) variant = fp32_pick_variant(g);
+    switch (variant) {
+        case 0: return launch_conv_cfg<V0>(g, cd, s);
+        case 1: return launch_conv_cfg<V1>(g, cd, s);
+        case 2: return launch_conv_cfg<V2>(g, cd, s);
+        case 3: return launch_conv_cfg<V3>(g, cd, s);
+        default: return fail(UGEMM_ERR_ARG, "conv: no variant %d", variant);
+    }
 }
 
 int launch_fp32(const Gemm<float> &g, int variant, hipStream_t s) {
