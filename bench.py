@@ -16,7 +16,9 @@ Also reported (one JSON line in total):
   * tall-skinny / short-wide SGEMM and the 4096^3 DGEMM (normal(0,1) inputs,
     DGEMM bit-compared across two runs);
   * 512 uneven im2col-shaped problems in one grouped persistent launch
-    (hipGraph replay) vs a loop of plain-kernel launches;
+    (hipGraph replay) vs a loop of plain-kernel launches, and the same 512
+    shapes as 3x3 convolutions with implicit im2col and the bias + LeakyReLU
+    epilogue;
   * the precision ladder at 8192^3 with entries sign * 2^e * (1+u), e in [-12, 12];
   * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
     into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread.
@@ -265,6 +267,81 @@ def config4(stream, fp32_single_tf):
     return r
 
 
+def config4_conv(stream, gemm_grouped_tf):
+    """Config 4 as convolutions: the same 512 draws (M = Cout, H = W, K = 9*Cin)
+    as 3x3, pad 1, stride 1 convolutions with bias + LeakyReLU, im2col
+    gathered inside the grouped kernel (no column matrix is stored).  The
+    GEMM shapes equal config 4's, so the rate next to config 4's grouped GEMM
+    rate is the cost of the implicit gather and the fused epilogue."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D04))
+    count = 512
+    Ms = 32 * g.integers(1, 17, size=count)
+    HW = g.choice([7, 14, 28, 56], size=count)
+    Ks = 144 * g.integers(1, 33, size=count)
+    Cin = Ks // 9
+    flop = float(np.sum(2.0 * Ms * HW * HW * Ks))
+    gi = np.random.Generator(np.random.PCG64(0x55474D4D14))
+    px, pw = uniform(gi, 16 << 20), uniform(gi, 4 << 20)
+    pb = uniform(gi, 1 << 12)
+    dpx, dpw, dpb = hip.DeviceArray.from_host(px), hip.DeviceArray.from_host(pw), hip.DeviceArray.from_host(pb)
+    o_off = np.concatenate([[0], np.cumsum(((Ms * HW * HW + 3) // 4) * 4)])
+    dO = hip.DeviceArray(int(o_off[-1]), np.float32)
+    x_off = 4 * gi.integers(0, ((16 << 20) - Cin * HW * HW) // 4)
+    w_off = 4 * gi.integers(0, ((4 << 20) - Ms * Ks) // 4)
+    b_off = gi.integers(0, (1 << 12) - Ms)
+    convs = [ug.ConvProblem(dpx.at(x_off[i]), int(Cin[i]), int(HW[i]), int(HW[i]), dpw.at(w_off[i]), 3, 1, 1,
+                            dO.at(o_off[i]), int(Ms[i]), dpb.at(b_off[i]), ug.ACT_LEAKY_RELU, 0.1)
+             for i in range(count)]
+    plan = ug.GroupedPlan(convs, capture=True)
+    tuned = plan.tune()
+    for _ in range(2):
+        plan.run(stream)
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    reps = 10
+    e0.record(stream)
+    for _ in range(reps):
+        plan.run(stream)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    # Check the smallest problems against a direct convolution (numpy, FP64).
+    out = dO.download()
+    worst = 0.0
+    for i in np.argsort(Ms * HW * HW * Ks)[:4]:
+        c, hw, m = int(Cin[i]), int(HW[i]), int(Ms[i])
+        x = px[x_off[i]:x_off[i] + c * hw * hw].reshape(c, hw, hw).astype(np.float64)
+        wk = pw[w_off[i]:w_off[i] + m * c * 9].reshape(m, c, 3, 3).astype(np.float64)
+        b = pb[b_off[i]:b_off[i] + m].astype(np.float64)
+        xp = np.zeros((c, hw + 2, hw + 2))
+        xp[:, 1:-1, 1:-1] = x
+        ref = np.zeros((m, hw, hw))
+        sc = np.zeros((m, hw, hw))
+        for a in range(3):
+            for bb in range(3):
+                win = xp[:, a:a + hw, bb:bb + hw]
+                ref += np.einsum("oc,cyx->oyx", wk[:, :, a, bb], win)
+                sc += np.einsum("oc,cyx->oyx", np.abs(wk[:, :, a, bb]), np.abs(win))
+        ref = ref.reshape(m, -1) + b[:, None]
+        ref = np.where(ref > 0, ref, 0.1 * ref)
+        sc = sc.reshape(m, -1) + np.abs(b)[:, None]
+        got = out[o_off[i]:o_off[i] + m * hw * hw].reshape(m, -1)
+        worst = max(worst, float(np.max(np.abs(got - ref) / (2 * (c * 9 + 1) * 2.0 ** -24 * sc))))
+    plan.destroy()
+    tf = flop / (ms * 1e-3) / 1e12
+    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.variants(ug.MODE_FP32)[tuned],
+         "grouped_graph_ms": round(ms, 4), "grouped_tflops": round(tf, 3),
+         "vs_grouped_gemm_same_shapes": round(tf / gemm_grouped_tf, 4) if gemm_grouped_tf else None,
+         "checked_problems_worst_err_over_bound": worst,
+         "note": "3x3 pad-1 convolutions + bias + LeakyReLU(0.1), implicit im2col; GEMM shapes of config 4"}
+    log(f"[config4-conv] grouped implicit conv {tf:.1f} TFLOP/s ({ms:.3f} ms)")
+    for d in (dpx, dpw, dpb, dO):
+        d.free()
+    e0.destroy()
+    e1.destroy()
+    return r
+
+
 def cpu_baseline(n, seconds_target):
     """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
     by the workload) on a row band of the n^3 workload, one thread."""
@@ -394,6 +471,7 @@ def main():
         line["config3"] = config3(stream, args.steps, args.warmup)
         fp32_tf = res.get("fp32", {}).get("tflops")
         line["config4_batched"] = config4(stream, fp32_tf)
+        line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)

@@ -149,6 +149,19 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
                                   int capture, void *stream);
 int ugemm_hip_grouped_run(ugemm_grouped_plan_t *plan, void *stream);
 void ugemm_hip_grouped_plan_destroy(ugemm_grouped_plan_t *plan);
+/* A plan of convolutions (each the operation of ugemm_hip_conv2d_dev:
+ * implicit im2col, bias, activation), run by the same persistent kernel in
+ * one launch -- the layers of a network, or the images of a batch.  The
+ * returned plan is run and destroyed with the functions above.  Buffers are
+ * device pointers and must stay valid while the plan is used. */
+typedef struct {
+    const float *in; int ich, w, h;              /* inputs  [ich][h][w] */
+    const float *weights; int k, pad, stride;    /* weights [ch][ich*k*k] */
+    float *out; int ch;                          /* outputs [ch][hcol*wcol] */
+    const float *bias; int act; float slope;     /* bias [ch] or NULL; UGEMM_ACT_* */
+} ugemm_conv_problem_t;
+int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
+                                       const ugemm_conv_problem_t *convs, int capture, void *stream);
 /* One-shot convenience: plan, run (no capture), destroy. */
 int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode,
                             void *stream);

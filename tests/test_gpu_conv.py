@@ -6,6 +6,8 @@ out = out > 0 ? out : 0.1 * out.  Acceptance: the FP32 elementwise bound
 2(K+1) eps32 * scale with scale = sum |w||col| + |bias| (the activation only
 shrinks errors).  The GEMM epilogue is checked against the oracle's exact
 product followed by the same bias/activation."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -166,3 +168,88 @@ def test_sgemm_epilogue_rejected_for_fp8():
     rc = L.ugemm_hip_sgemm_ex_dev(b"R", b"N", b"N", 4, 4, 4, 1.0, d.ptr, 4, d.ptr, 4, 0.0, d.ptr, 4,
                                   ug.MODE_FP32, d.ptr, 5, 1, 0.1, None)
     assert rc == ug.ERR_ARG
+
+
+# ---------------------------------------------------------------------------
+# Grouped convolutions: one persistent launch over many independent layers.
+
+def _conv_batch(n, seed):
+    g = orc.rng(seed)
+    out = []
+    for i in range(n):
+        ich = int(g.choice([1, 3, 8, 16, 24]))
+        h = int(g.choice([5, 7, 13, 14, 20]))
+        w = h + int(g.integers(0, 3))
+        k = int(g.choice([1, 3, 3, 5]))
+        pad = int(g.integers(0, k // 2 + 1))
+        stride = int(g.choice([1, 1, 2]))
+        ch = int(g.choice([4, 16, 33, 64]))
+        x = orc.uniform(g, (ich, h, w))
+        wts = orc.uniform(g, (ch, ich * k * k))
+        bias = orc.uniform(g, (ch,)) if i % 3 else None
+        act = ug.ACT_LEAKY_RELU if i % 2 else ug.ACT_NONE
+        out.append((ich, h, w, k, pad, stride, ch, x, wts, bias, act))
+    return out
+
+
+def _conv_plan(layers, capture):
+    keep, probs, outs = [], [], []
+    for (ich, h, w, k, pad, stride, ch, x, wts, bias, act) in layers:
+        hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
+        d_x, d_w = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(wts)
+        d_b = hip.DeviceArray.from_host(bias) if bias is not None else None
+        d_o = hip.DeviceArray.from_host(np.full((ch, hcol * wcol), np.nan, np.float32))
+        keep += [d_x, d_w, d_b]
+        outs.append(d_o)
+        probs.append(ug.ConvProblem(d_x.ptr, ich, w, h, d_w.ptr, k, pad, stride, d_o.ptr, ch,
+                                    d_b.ptr if d_b is not None else None, act, 0.1))
+    return ug.GroupedPlan(probs, capture=capture), outs, keep
+
+
+@pytest.mark.parametrize("capture", [False, True])
+def test_grouped_convolutions_match_oracle(capture):
+    layers = _conv_batch(24, 1234)
+    plan, outs, keep = _conv_plan(layers, capture)
+    s = hip.Stream()
+    first = None
+    for r in range(2):
+        plan.run(s)
+        s.synchronize()
+        now = [o.download() for o in outs]
+        if first is None:
+            first = now
+        else:
+            assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(first, now))
+    for (ich, h, w, k, pad, stride, ch, x, wts, bias, act), got in zip(layers, first):
+        exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=act, slope=0.1)
+        ok, worst = orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))
+        assert ok, ((ich, h, w, k, pad, stride, ch), worst)
+    plan.destroy()
+    s.destroy()
+
+
+@pytest.mark.parametrize("variant", range(4))
+def test_grouped_convolutions_every_variant(variant):
+    layers = _conv_batch(10, 4321 + variant)
+    ug.force_variant(ug.MODE_FP32, variant)
+    try:
+        plan, outs, keep = _conv_plan(layers, False)
+    finally:
+        ug.force_variant(ug.MODE_FP32, -1)
+    plan.run(None)
+    hip.synchronize()
+    for (ich, h, w, k, pad, stride, ch, x, wts, bias, act), o in zip(layers, outs):
+        exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=act, slope=0.1)
+        assert orc.check_bound(o.download(), exact, scale, orc.tolerance(ich * k * k))[0]
+    plan.destroy()
+
+
+def test_grouped_convolution_errors():
+    L = ug.load()
+    h = ctypes.c_void_p()
+    bad = ug.ConvProblem(16, 1, 4, 4, 16, 5, 0, 1, 16, 1, None, 1, 0.1)      # kernel larger than the image
+    arr = (ug.ConvProblem * 1)(bad)
+    assert L.ugemm_hip_conv_grouped_plan_create(ctypes.byref(h), 1, arr, 0, None) == ug.ERR_ARG
+    bad = ug.ConvProblem(None, 1, 4, 4, 16, 3, 0, 1, 16, 1, None, 1, 0.1)    # NULL input
+    arr = (ug.ConvProblem * 1)(bad)
+    assert L.ugemm_hip_conv_grouped_plan_create(ctypes.byref(h), 1, arr, 0, None) == ug.ERR_ARG

@@ -46,6 +46,14 @@ class Problem(ctypes.Structure):
                 ("C", ctypes.c_void_p), ("ldc", ctypes.c_int)]
 
 
+class ConvProblem(ctypes.Structure):
+    """ugemm_conv_problem_t (include/ugemm_hip.h)."""
+    _fields_ = [("in_", ctypes.c_void_p), ("ich", ctypes.c_int), ("w", ctypes.c_int), ("h", ctypes.c_int),
+                ("weights", ctypes.c_void_p), ("k", ctypes.c_int), ("pad", ctypes.c_int), ("stride", ctypes.c_int),
+                ("out", ctypes.c_void_p), ("ch", ctypes.c_int),
+                ("bias", ctypes.c_void_p), ("act", ctypes.c_int), ("slope", ctypes.c_float)]
+
+
 def load(path=None):
     """Loads libugemm_hip.so (once) and declares every exported signature."""
     global _lib
@@ -71,6 +79,7 @@ def load(path=None):
         "convolution_LReLU_hip": (None, [vp, i, i, i, vp, i, i, i, vp, i, vp]),
         "ugemm_hip_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(Problem), i, i, vp]),
         "ugemm_hip_grouped_run": (i, [vp, vp]),
+        "ugemm_hip_conv_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(ConvProblem), i, vp]),
         "ugemm_hip_grouped_plan_destroy": (None, [vp]),
         "ugemm_hip_sgemm_grouped": (i, [i, ctypes.POINTER(Problem), i, vp]),
         "ugemm_hip_last_error": (i, []),
@@ -210,14 +219,21 @@ def force_variant(mode, variant):
 
 
 class GroupedPlan:
-    """A planned grouped SGEMM (one persistent launch, optionally a hipGraph)."""
+    """A planned grouped SGEMM (one persistent launch, optionally a hipGraph).
+    A list of ConvProblem makes a plan of convolutions instead."""
 
     def __init__(self, problems, mode=MODE_FP32, capture=True, stream=None):
-        self._arr = (Problem * len(problems))(*problems)
         h = ctypes.c_void_p()
         s = stream.handle if hasattr(stream, "handle") else stream
-        _raise_if(load().ugemm_hip_grouped_plan_create(ctypes.byref(h), len(problems), self._arr, mode,
-                                                       1 if capture else 0, s))
+        if problems and isinstance(problems[0], ConvProblem):
+            self._arr = (ConvProblem * len(problems))(*problems)
+            rc = load().ugemm_hip_conv_grouped_plan_create(ctypes.byref(h), len(problems), self._arr,
+                                                          1 if capture else 0, s)
+        else:
+            self._arr = (Problem * len(problems))(*problems)
+            rc = load().ugemm_hip_grouped_plan_create(ctypes.byref(h), len(problems), self._arr, mode,
+                                                      1 if capture else 0, s)
+        _raise_if(rc)
         self.handle = h.value
 
     def run(self, stream=None):

@@ -81,7 +81,7 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
 
     // Implicit im2col: thread t always holds column nl = t % BN of the B
     // tile (NT is a multiple of BN) and rows kl = t / BN + i * (NT / BN).
-    static_assert(!CONV || (NT % BN == 0 && !TB), "implicit operand layout");
+    static_assert(!CONV || (NT % BN == 0 && BN % 64 == 0 && !TB), "implicit operand layout");
     constexpr int CONV_N = CONV ? BK * BN / NT : 1;
     float cvals[CONV_N];
     long c_base = 0;
@@ -104,7 +104,9 @@ __device__ __forceinline__ void fp32_tile(const int M, const int N, const int K,
         if constexpr (CONV) {
 #pragma unroll
             for (int i = 0; i < CONV_N; ++i) {
-                const int k = k0 + t / BN + i * (NT / BN);
+                // k is the same for the 64 lanes of a wave (BN is a multiple
+                // of 64): one scalar load of the table entry per wave.
+                const int k = __builtin_amdgcn_readfirstlane(k0 + t / BN + i * (NT / BN));
                 float v = 0.0f;
                 if (c_ncol && k < K) {
                     const int2 e = cd.ktab[k];

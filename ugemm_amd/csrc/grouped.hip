@@ -43,7 +43,10 @@ struct DevProblem {
     long lda, ldb, ldc;
     int M, N, K;          // K is 0 when alpha == 0 (A and B are then not read)
     float alpha, beta;
-    int layout;           // bit 0: A transposed, bit 1: B transposed, bit 2: A and B allow 128-bit loads
+    int layout;           // bit 0: A transposed, bit 1: B transposed, bit 2: A and B allow 128-bit
+                          // loads, bit 3: B is the implicit im2col operand cd (convolution)
+    Epilogue epi;         // bias + activation (convolutions)
+    ConvDesc cd;
 };
 
 struct DevTile {
@@ -58,7 +61,8 @@ __global__ void grouped_reset_heads(int *heads) {
 
 // At least 3 waves per SIMD (<= 168 VGPRs): the layout switch would
 // otherwise push the 128 x 128 variant just past that limit.
-template <class CF>
+// CONV: a plan of convolutions (every problem has layout bit 3), else of GEMMs.
+template <class CF, bool CONV>
 __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) void grouped_fp32_kernel(
     const DevProblem *__restrict__ probs, const DevTile *__restrict__ tiles, const int *__restrict__ q_begin,
     int *heads) {
@@ -82,6 +86,12 @@ __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) voi
         const DevTile tl = tiles[i];
         const DevProblem p = probs[tl.problem];
         const bool aligned = (p.layout & 4) != 0;
+        if constexpr (CONV) {
+            fp32_tile<CF, false, false, 4, true>(p.M, p.N, p.K, p.alpha, p.A, p.lda, nullptr, 0, p.beta, p.C, p.ldc,
+                                                 tl.m0, tl.n0, lds, aligned, p.epi, p.cd);
+            __syncthreads();
+            continue;
+        }
         switch (p.layout & 3) {
             case 0:
                 fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
@@ -112,13 +122,14 @@ constexpr int kNumVariants = 4;
 
 struct VariantOps {
     int bm, bn, bk;
-    const void *kern;
+    const void *kern, *kern_conv;
     int nt;
 };
 
 template <class CF>
 VariantOps ops_of() {
-    return VariantOps{CF::BM, CF::BN, CF::BK, (const void *)grouped_fp32_kernel<CF>, CF::NT};
+    return VariantOps{CF::BM, CF::BN, CF::BK, (const void *)grouped_fp32_kernel<CF, false>,
+                      (const void *)grouped_fp32_kernel<CF, true>, CF::NT};
 }
 
 VariantOps variant_ops(int v) {
@@ -147,6 +158,8 @@ struct ugemm_grouped_plan {
     int *d_queues = nullptr;   // kQueues + 1 queue starts
     int *d_heads = nullptr;    // kQueues queue heads (zeroed before every launch)
     bool capture = false;
+    bool conv = false;          // a plan of convolutions (grouped_fp32_kernel<CF, true>)
+    int2 *d_ktab = nullptr;     // the convolutions' im2col offset tables, one block per problem
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -175,7 +188,7 @@ int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (e != hipSuccess) return hip_fail(e, "grouped queue reset");
     void *args[] = {&p->d_probs, &p->d_tiles, &p->d_queues, &p->d_heads};
     timing_begin(s);
-    e = hipLaunchKernel(ops.kern, dim3(p->grid), dim3(ops.nt), args, 0, s);
+    e = hipLaunchKernel(p->conv ? ops.kern_conv : ops.kern, dim3(p->grid), dim3(ops.nt), args, 0, s);
     timing_end(s);
     if (e != hipSuccess) return hip_fail(e, "grouped launch");
     return UGEMM_OK;
@@ -209,7 +222,9 @@ int build_schedule(ugemm_grouped_plan *p, int v) {
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile &a, const HostTile &b) { return a.cost > b.cost; });
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ops.kern, ops.nt, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->conv ? ops.kern_conv : ops.kern, ops.nt, 0) !=
+            hipSuccess ||
+        per_cu < 1)
         per_cu = 1;
     const long slots = (long)device_cu_count() * per_cu;
     const int G = (int)std::min<long>((long)tiles.size(), slots);
@@ -266,6 +281,38 @@ int default_variant(const ugemm_grouped_plan *p) {
     return (all == 0 || big >= 0.5 * all) ? 0 : 1;
 }
 
+// Uploads the problem table, picks the tile variant (forced, tuned, or by
+// default), builds the queues and, if asked, captures the launch.  Takes
+// ownership of p: on failure it is destroyed.
+int finish_plan(ugemm_grouped_plan *p, int count, int capture, ugemm_grouped_plan_t **plan) {
+    p->capture = capture != 0;
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_probs, std::max<size_t>(1, p->probs.size()) * sizeof(DevProblem))) != hipSuccess ||
+        (count > 0 && (e = hipMemcpy(p->d_probs, p->probs.data(), p->probs.size() * sizeof(DevProblem),
+                                     hipMemcpyHostToDevice)) != hipSuccess)) {
+        ugemm_hip_grouped_plan_destroy(p);
+        return hip_fail(e, "grouped: descriptor upload");
+    }
+    if (p->capture) {
+        // Capture needs a non-null stream; an owned one is used for capture
+        // only (replays go to the stream passed to ugemm_hip_grouped_run).
+        if ((e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
+            ugemm_hip_grouped_plan_destroy(p);
+            return hip_fail(e, "grouped: capture stream");
+        }
+    }
+    int v = forced_variant(UGEMM_MODE_FP32);
+    if (v < 0) v = tuned_variant(TUNE_KEY_GROUPED, p->conv, false, count, (int)std::min(p->flops / 1e6, 2e9), 0);
+    if (v < 0) v = default_variant(p);
+    const int rc = build_schedule(p, v);
+    if (rc) {
+        ugemm_hip_grouped_plan_destroy(p);
+        return rc;
+    }
+    *plan = p;
+    return UGEMM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,33 +355,72 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
         p->probs.push_back(d);
         p->flops += 2.0 * q.M * q.N * q.K;
     }
-    p->capture = capture != 0;
-    hipError_t e;
-    if ((e = hipMalloc(&p->d_probs, std::max<size_t>(1, p->probs.size()) * sizeof(DevProblem))) != hipSuccess ||
-        (count > 0 && (e = hipMemcpy(p->d_probs, p->probs.data(), p->probs.size() * sizeof(DevProblem),
-                                     hipMemcpyHostToDevice)) != hipSuccess)) {
-        ugemm_hip_grouped_plan_destroy(p);
-        return hip_fail(e, "grouped: descriptor upload");
+    return finish_plan(p, count, capture, plan);
+}
+
+int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const ugemm_conv_problem_t *convs,
+                                       int capture, void *stream) {
+    if (!plan) return fail(UGEMM_ERR_ARG, "grouped conv: NULL plan pointer");
+    *plan = nullptr;
+    if (count < 0 || (count > 0 && !convs)) return fail(UGEMM_ERR_ARG, "grouped conv: bad problem list");
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::vector<int> hcol(count), wcol(count);
+    long ktab_total = 0;
+    for (int i = 0; i < count; ++i) {
+        const ugemm_conv_problem_t &c = convs[i];
+        rc = conv_shape(c.ich, c.w, c.h, c.k, c.pad, c.stride, c.ch, &hcol[i], &wcol[i]);
+        if (rc) return rc;
+        if (!c.in || !c.weights || !c.out) return fail(UGEMM_ERR_ARG, "grouped conv: problem %d: NULL buffer", i);
+        if (c.act != UGEMM_ACT_NONE && c.act != UGEMM_ACT_LEAKY_RELU)
+            return fail(UGEMM_ERR_ARG, "grouped conv: problem %d: bad activation %d", i, c.act);
+        ktab_total += (long)c.ich * c.k * c.k;
     }
-    if (p->capture) {
-        // Capture needs a non-null stream; an owned one is used for capture
-        // only (replays go to the stream passed to ugemm_hip_grouped_run).
-        if ((e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
-            ugemm_hip_grouped_plan_destroy(p);
-            return hip_fail(e, "grouped: capture stream");
+    // im2col offset tables: k = (ch*kh + i)*kw + j -> {ch*H*W + i*W + j, (i << 16) | j}.
+    std::vector<int2> ktab((size_t)std::max<long>(1, ktab_total));
+    std::vector<long> kofs(count);
+    long at = 0;
+    for (int i = 0; i < count; ++i) {
+        const ugemm_conv_problem_t &c = convs[i];
+        kofs[i] = at;
+        for (int q = 0; q < c.ich * c.k * c.k; ++q) {
+            const int ci = q / (c.k * c.k), r = q - ci * c.k * c.k, ki = r / c.k, kj = r - ki * c.k;
+            ktab[at++] = make_int2(ci * c.h * c.w + ki * c.w + kj, (ki << 16) | kj);
         }
     }
-    (void)stream;
-    int v = forced_variant(UGEMM_MODE_FP32);
-    if (v < 0) v = tuned_variant(TUNE_KEY_GROUPED, false, false, count, (int)std::min(p->flops / 1e6, 2e9), 0);
-    if (v < 0) v = default_variant(p);
-    rc = build_schedule(p, v);
-    if (rc) {
+    ugemm_grouped_plan *p = new ugemm_grouped_plan();
+    p->mode = UGEMM_MODE_FP32;
+    p->conv = true;
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_ktab, ktab.size() * sizeof(int2))) != hipSuccess ||
+        (e = hipMemcpy(p->d_ktab, ktab.data(), ktab.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) {
         ugemm_hip_grouped_plan_destroy(p);
-        return rc;
+        return hip_fail(e, "grouped conv: offset tables");
     }
-    *plan = p;
-    return UGEMM_OK;
+    for (int i = 0; i < count; ++i) {
+        const ugemm_conv_problem_t &c = convs[i];
+        const int K = c.ich * c.k * c.k, N = hcol[i] * wcol[i];
+        DevProblem d{};
+        d.A = c.weights;
+        d.lda = K;
+        d.C = c.out;
+        d.ldc = N;
+        d.M = c.ch;
+        d.N = N;
+        d.K = K;
+        d.alpha = 1.0f;
+        d.beta = 0.0f;
+        d.layout = 8 | ((ugemm_aligned16(c.weights) && K % 4 == 0) ? 4 : 0);
+        d.epi.bias = c.bias;
+        d.epi.bias_rows = 1;
+        d.epi.act = c.act;
+        d.epi.slope = c.slope;
+        d.cd = ConvDesc{c.in, p->d_ktab + kofs[i], c.h, c.w, wcol[i], c.stride, c.pad};
+        p->probs.push_back(d);
+        p->flops += 2.0 * c.ch * N * K;
+    }
+    (void)stream;
+    return finish_plan(p, count, capture, plan);
 }
 
 int ugemm_hip_grouped_run(ugemm_grouped_plan_t *p, void *stream) {
@@ -352,6 +438,7 @@ void ugemm_hip_grouped_plan_destroy(ugemm_grouped_plan_t *p) {
     if (p->cap_stream) hipStreamSynchronize(p->cap_stream);
     release_schedule(p);
     hipFree(p->d_probs);
+    hipFree(p->d_ktab);
     if (p->cap_stream) hipStreamDestroy(p->cap_stream);
     delete p;
 }
@@ -408,7 +495,7 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     hipStreamSynchronize(s);
     hipStreamDestroy(s);
     if (rc) return rc;
-    tune_record(TUNE_KEY_GROUPED, false, false, (int)p->probs.size(), (int)std::min(p->flops / 1e6, 2e9), 0, best,
+    tune_record(TUNE_KEY_GROUPED, p->conv, false, (int)p->probs.size(), (int)std::min(p->flops / 1e6, 2e9), 0, best,
                 best_ms / 3);
     return best;
 }

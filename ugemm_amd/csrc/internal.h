@@ -87,6 +87,9 @@ int launch_dgemm(const Gemm<double> &g, int variant, hipStream_t s);
 // followed by the epilogue when one is given.
 int launch_scale_f32(int M, int N, float beta, float *C, long ldc, hipStream_t s, const Epilogue &epi = Epilogue());
 
+// Validates a convolution's shape and returns its output extent.
+int conv_shape(int ich, int w, int h, int k, int pad, int stride, int ch, int *hcol, int *wcol);
+
 // Implicit-GEMM convolution (FP32 tile kernel): g.B is unused, opB comes from cd.
 int launch_conv_fp32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s);
 int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s);

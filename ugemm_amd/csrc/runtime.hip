@@ -352,7 +352,7 @@ static int env_mode() {
 }
 
 // Implicit-GEMM convolution: M = ch, N = hcol * wcol, K = ich * k * k.
-static int conv_shape(int ich, int w, int h, int k, int pad, int stride, int ch, int *hcol, int *wcol) {
+int conv_shape(int ich, int w, int h, int k, int pad, int stride, int ch, int *hcol, int *wcol) {
     if (ich < 1 || w < 1 || h < 1 || k < 1 || ch < 1 || stride < 1 || pad < 0 || k > 32767)
         return fail(UGEMM_ERR_ARG, "conv: bad shape ich=%d w=%d h=%d k=%d pad=%d stride=%d ch=%d", ich, w, h, k, pad,
                     stride, ch);
