@@ -20,6 +20,7 @@ Also reported (one JSON line in total):
     shapes as 3x3 convolutions with implicit im2col and the bias + LeakyReLU
     epilogue;
   * the precision ladder at 8192^3 with entries sign * 2^e * (1+u), e in [-12, 12];
+  * the BLAS-1/2 helpers (axpy, gemv) against the HBM roofline;
   * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
     into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread.
 """
@@ -342,6 +343,55 @@ def config4_conv(stream, gemm_grouped_tf):
     return r
 
 
+def blas12(stream):
+    """BLAS-1/2 helpers (ugemm.h:76-160) against the HBM roofline: axpy moves
+    3 words per element (read x, read y, write y); gemv reads A once, x once
+    and y once and writes y."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D07))
+    res = {}
+
+    def run(fn, reps=10):
+        fn()
+        stream.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        e0.destroy()
+        e1.destroy()
+        return ms
+
+    for name, dt, n in (("saxpy", np.float32, 1 << 28), ("daxpy", np.float64, 1 << 27)):
+        x = hip.DeviceArray.from_host(g.standard_normal(n).astype(dt))
+        y = hip.DeviceArray.from_host(g.standard_normal(n).astype(dt))
+        ms = run(lambda: ug.axpy_dev(n, 0.5, x, 1, y, 1, double=dt == np.float64, stream=stream))
+        gbs = 3.0 * n * np.dtype(dt).itemsize / (ms * 1e-3) / 1e9
+        res[f"{name}_{n}"] = {"ms": round(ms, 4), "GB_per_s": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_GBPS, 4)}
+        log(f"[blas12] {name} N={n}: {gbs:.0f} GB/s")
+        x.free()
+        y.free()
+    for name, dt, M, N in (("sgemv", np.float32, 16384, 16384), ("dgemv", np.float64, 16384, 8192)):
+        A = hip.DeviceArray.from_host(g.standard_normal(M * N).astype(dt))
+        for trans in "NT":
+            rows, cols = (M, N) if trans == "N" else (N, M)
+            x = hip.DeviceArray.from_host(g.standard_normal(N).astype(dt))
+            y = hip.DeviceArray.from_host(g.standard_normal(M).astype(dt))
+            ms = run(lambda: ug.gemv_dev(trans, M, N, 1.0, A, rows, x, 1, 0.5, y, 1, double=dt == np.float64,
+                                         stream=stream))
+            nbytes = (M * N + N + 2 * M) * np.dtype(dt).itemsize
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            res[f"{name}_{trans}_{M}x{N}"] = {"ms": round(ms, 4), "GB_per_s": round(gbs, 1),
+                                              "frac_of_hbm_peak": round(gbs / HBM_GBPS, 4)}
+            log(f"[blas12] {name} {trans} {M}x{N}: {gbs:.0f} GB/s")
+            x.free()
+            y.free()
+        A.free()
+    return res
+
+
 def cpu_baseline(n, seconds_target):
     """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
     by the workload) on a row band of the n^3 workload, one thread."""
@@ -472,6 +522,7 @@ def main():
         fp32_tf = res.get("fp32", {}).get("tflops")
         line["config4_batched"] = config4(stream, fp32_tf)
         line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
+        line["blas12"] = blas12(stream)
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)

@@ -167,7 +167,35 @@ int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode
                             void *stream);
 
 /* ------------------------------------------------------------------ */
-/* 5. Status and tuning.                                              */
+/* 5. BLAS-1/2 helpers (HBM-bound kernels).                           */
+/*    saxpy/daxpy: y[n*incy] += alpha * x[n*incx], n < N               */
+/*      (saxpy_cpu / daxpy_cpu, ugemm.h:76-101).                       */
+/*    sgemv/dgemv, the element mapping of sgemv_cpu / dgemv_cpu        */
+/*    (ugemm.h:103-160), m < M, n < N:                                  */
+/*      'N': y[m*incy] = alpha*sum_n A[m + n*lda]*x[n*incx] + beta*y   */
+/*      other: y[m*incy] = alpha*sum_n A[n + m*lda]*x[n*incx] + beta*y */
+/*    x is read with incx (the reference reads it with incy,            */
+/*    ugemm.h:116,147 -- identical when incx == incy).  Increments      */
+/*    must be >= 1; beta == 0 does not read y.                          */
+/*    The *_hip functions have the reference's argument lists (host     */
+/*    pointers, synchronous, status in ugemm_hip_last_error()); the     */
+/*    _dev functions take device pointers and a stream.                 */
+/* ------------------------------------------------------------------ */
+void saxpy_hip(int N, float alpha, const float *x, int incx, float *y, int incy);
+void daxpy_hip(int N, double alpha, const double *x, int incx, double *y, int incy);
+void sgemv_hip(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx,
+               float beta, float *y, int incy);
+void dgemv_hip(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
+               double beta, double *y, int incy);
+int ugemm_hip_saxpy_dev(int N, float alpha, const float *dx, int incx, float *dy, int incy, void *stream);
+int ugemm_hip_daxpy_dev(int N, double alpha, const double *dx, int incx, double *dy, int incy, void *stream);
+int ugemm_hip_sgemv_dev(char trans, int M, int N, float alpha, const float *dA, int lda, const float *dx, int incx,
+                        float beta, float *dy, int incy, void *stream);
+int ugemm_hip_dgemv_dev(char trans, int M, int N, double alpha, const double *dA, int lda, const double *dx,
+                        int incx, double beta, double *dy, int incy, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* 6. Status and tuning.                                              */
 /* ------------------------------------------------------------------ */
 int ugemm_hip_last_error(void);
 const char *ugemm_hip_last_error_message(void);

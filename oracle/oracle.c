@@ -372,3 +372,39 @@ int oracle_conv_lrelu(const float *inputs, int ich, int w, int h, const float *w
 	free(col);
 	return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * BLAS-1/2 helpers (ugemm.h:76-160), restated in the working precision:
+ *   oracle_saxpy / oracle_daxpy: y[n*incy] += alpha * x[n*incx]   (ugemm.h:76-101)
+ *   oracle_sgemv / oracle_dgemv: sgemv_cpu / dgemv_cpu (ugemm.h:103-160),
+ *     y[m*incy] = alpha * sum_n a(m, n) * x[n*xinc] + beta * y[m*incy],
+ *     a(m, n) = trans == 'N' ? A[m + n*lda] : A[n + m*lda], the sum taken
+ *     sequentially over n.  x_by_incy != 0 reproduces the reference exactly,
+ *     which reads x with incy (xinc = incy, ugemm.h:116,124,147,155); 0 gives
+ *     the intended xinc = incx.  The reference also reads y when beta == 0;
+ *     so does this restatement.
+ * ------------------------------------------------------------------------ */
+#define DEFINE_AXPY(T, NAME)                                                              \
+void NAME(int N, T alpha, const T *x, int incx, T *y, int incy)                          \
+{                                                                                         \
+	for (int n = 0; n < N; n++) y[(long)n * incy] += alpha * x[(long)n * incx];          \
+}
+DEFINE_AXPY(float, oracle_saxpy)
+DEFINE_AXPY(double, oracle_daxpy)
+
+#define DEFINE_GEMV(T, NAME)                                                                  \
+void NAME(char trans, int M, int N, T alpha, const T *A, int lda, const T *x, int incx,      \
+          T beta, T *y, int incy, int x_by_incy)                                              \
+{                                                                                             \
+	const long xinc = x_by_incy ? incy : incx;                                               \
+	for (int m = 0; m < M; m++) {                                                            \
+		T sum = 0;                                                                           \
+		for (int n = 0; n < N; n++) {                                                        \
+			const T a = trans == 'N' ? A[m + (long)n * lda] : A[n + (long)m * lda];          \
+			sum += a * x[n * xinc];                                                          \
+		}                                                                                    \
+		y[(long)m * incy] = alpha * sum + beta * y[(long)m * incy];                          \
+	}                                                                                        \
+}
+DEFINE_GEMV(float, oracle_sgemv)
+DEFINE_GEMV(double, oracle_dgemv)

@@ -51,6 +51,14 @@ def lib():
             fn = getattr(L, name)
             fn.restype = i
             fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp, i, vp, vp, i]
+        L.oracle_saxpy.restype = None
+        L.oracle_saxpy.argtypes = [i, f, vp, i, vp, i]
+        L.oracle_daxpy.restype = None
+        L.oracle_daxpy.argtypes = [i, d, vp, i, vp, i]
+        L.oracle_sgemv.restype = None
+        L.oracle_sgemv.argtypes = [c, i, i, f, vp, i, vp, i, f, vp, i, i]
+        L.oracle_dgemv.restype = None
+        L.oracle_dgemv.argtypes = [c, i, i, d, vp, i, vp, i, d, vp, i, i]
         L.oracle_im2col.restype = None
         L.oracle_im2col.argtypes = [vp, i, i, i, i, i, i, i, i, i, vp]
         L.oracle_conv_lrelu.restype = i
@@ -75,6 +83,14 @@ def ref_lib():
             fn = getattr(L, name)
             fn.restype = None
             fn.argtypes = [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i]
+        L.saxpy_cpu.restype = None
+        L.saxpy_cpu.argtypes = [i, f, vp, i, vp, i]
+        L.daxpy_cpu.restype = None
+        L.daxpy_cpu.argtypes = [i, d, vp, i, vp, i]
+        L.sgemv_cpu.restype = None
+        L.sgemv_cpu.argtypes = [c, i, i, f, vp, i, vp, i, f, vp, i]
+        L.dgemv_cpu.restype = None
+        L.dgemv_cpu.argtypes = [c, i, i, d, vp, i, vp, i, d, vp, i]
         _r = L
     return _r
 
@@ -353,3 +369,27 @@ def conv_lrelu(inputs, ich, w, h, weights, k, pad, stride, ch, bias=None, act=1,
     if rc:
         raise MemoryError("oracle_conv_lrelu")
     return out, scale
+
+
+# ---------------------------------------------------------------------------
+# BLAS-1/2 (ugemm.h:76-160): bindings to oracle.c and dense views.
+
+def axpy(N, alpha, x, incx, y, incy):
+    """y updated in place (working precision of y's dtype)."""
+    fn = lib().oracle_daxpy if y.dtype == np.float64 else lib().oracle_saxpy
+    fn(N, float(alpha), _p(x), incx, _p(y), incy)
+
+
+def gemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy, x_by_incy=False):
+    """y updated in place; x_by_incy reproduces the reference's x indexing."""
+    fn = lib().oracle_dgemv if y.dtype == np.float64 else lib().oracle_sgemv
+    fn(_c(trans), M, N, float(alpha), _p(A), lda, _p(x), incx, float(beta), _p(y), incy, 1 if x_by_incy else 0)
+
+
+def gemv_dense(trans, M, N, A, lda, x, incx):
+    """(a, xv): the M x N matrix a(m, n) and the N-vector x[n*incx] that the
+    gemv of ugemm.h:103-160 multiplies (for an accurate reference)."""
+    A = np.asarray(A).ravel()
+    m, n = np.arange(M)[:, None], np.arange(N)[None, :]
+    idx = (m + n * lda) if trans == "N" else (n + m * lda)
+    return A[idx], np.asarray(x).ravel()[np.arange(N) * incx]

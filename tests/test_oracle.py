@@ -365,3 +365,59 @@ def test_im2col_known_answer():
     # pad 1, stride 2, 3x3 kernel: the corner taps of the first pixel are outside the image
     col = orc.im2col(x, 1, 3, 3, 3, 1, 2)
     assert col.shape == (9, 4) and col[0, 0] == 0 and col[4, 0] == 1 and col[8, 3] == 0 and col[4, 3] == 9
+
+
+# ---------------------------------------------------------------------------
+# BLAS-1/2 oracle vs the compiled reference's saxpy/daxpy/sgemv/dgemv.
+
+def _blas_golden():
+    import ast
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_blas12.npz"))
+    return d, [ast.literal_eval(s) for s in d["index"]]
+
+
+def test_blas12_oracle_matches_reference_golden():
+    d, index = _blas_golden()
+    assert len(index) == 224
+    diverging = 0
+    for (key, name, N, incx, incy, alpha, beta, trans, M, lda, xbug) in index:
+        dt = np.float64 if name[0] == "d" else np.float32
+        eps = orc.EPS64 if dt == np.float64 else orc.EPS32
+        x, y0, yref = d[key + "_x"], d[key + "_y0"], d[key + "_y"]
+        y = y0.copy()
+        if "axpy" in name:
+            orc.axpy(N, alpha, x, incx, y, incy)
+            # y + alpha*x: one or two roundings (the reference may contract to an FMA)
+            sc = np.abs(y0.astype(np.float64)).copy()
+            sc[::incy][:N] += abs(alpha) * np.abs(x[::incx][:N].astype(np.float64))
+            assert np.all(np.abs(y.astype(np.float64) - yref) <= 2 * eps * sc), key
+            continue
+        A = d[key + "_A"]
+        orc.gemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy, x_by_incy=True)
+        a, xv = orc.gemv_dense(trans, M, N, A, lda, x, incy)        # the reference's reading of x
+        scale = abs(alpha) * (np.abs(a) @ np.abs(xv)) + abs(beta) * np.abs(y0[::incy][:M].astype(np.float64))
+        tol = 2 * (N + 2) * eps * scale
+        assert np.all(np.abs(y[::incy][:M] - yref[::incy][:M]) <= tol), key
+        if xbug and N > 1 and alpha != 0:
+            y2 = y0.copy()
+            orc.gemv(trans, M, N, alpha, A, lda, x, incx, beta, y2, incy, x_by_incy=False)
+            diverging += not np.allclose(y2, yref, rtol=1e-3)
+    assert diverging > 0   # the reference's x-indexing (incy) really differs from incx
+
+
+def test_blas12_oracle_matches_numpy():
+    g = orc.rng(1212)
+    for trans in "NT":
+        M, N, pad, incx, incy = 37, 53, 2, 3, 2
+        rows, cols = (M, N) if trans == "N" else (N, M)
+        lda = rows + pad
+        A = orc.uniform(g, (cols * lda,), dtype=np.float64)
+        x = orc.uniform(g, ((N - 1) * incx + 1,), dtype=np.float64)
+        y0 = orc.uniform(g, ((M - 1) * incy + 1,), dtype=np.float64)
+        y = y0.copy()
+        orc.gemv(trans, M, N, 1.5, A, lda, x, incx, 0.5, y, incy)
+        a = A.reshape(cols, lda)[:, :rows]          # column-major rows x cols, stored by columns
+        a = a.T if trans == "N" else a              # 'N': a(m, n) = A[m + n*lda]; 'T': A[n + m*lda]
+        want = 1.5 * a @ x[::incx][:N] + 0.5 * y0[::incy][:M]
+        assert np.allclose(y[::incy][:M], want, rtol=1e-13, atol=1e-13)
+        assert np.array_equal(y[1::incy], y0[1::incy])          # gaps untouched

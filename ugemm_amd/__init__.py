@@ -94,6 +94,14 @@ def load(path=None):
         "ugemm_hip_variant_count": (i, [i]),
         "ugemm_hip_variant_name": (cs, [i, i]),
         "ugemm_hip_version": (cs, []),
+        "saxpy_hip": (None, [i, f, vp, i, vp, i]),
+        "daxpy_hip": (None, [i, d, vp, i, vp, i]),
+        "sgemv_hip": (None, [c, i, i, f, vp, i, vp, i, f, vp, i]),
+        "dgemv_hip": (None, [c, i, i, d, vp, i, vp, i, d, vp, i]),
+        "ugemm_hip_saxpy_dev": (i, [i, f, vp, i, vp, i, vp]),
+        "ugemm_hip_daxpy_dev": (i, [i, d, vp, i, vp, i, vp]),
+        "ugemm_hip_sgemv_dev": (i, [c, i, i, f, vp, i, vp, i, f, vp, i, vp]),
+        "ugemm_hip_dgemv_dev": (i, [c, i, i, d, vp, i, vp, i, d, vp, i, vp]),
         "ugemm_hip_timing_enable": (i, [i]),
         "ugemm_hip_timing_collect": (i, [ctypes.POINTER(ctypes.c_double)]),
     }
@@ -193,6 +201,36 @@ def convolution_LReLU(inputs, ich, w, h, weights, k, pad, stride, outputs, ch, b
     L = load()
     L.convolution_LReLU_hip(_ptr(inputs), ich, w, h, _ptr(weights), k, pad, stride, _ptr(outputs), ch, _ptr(bias))
     return _raise_if(L.ugemm_hip_last_error())
+
+
+def axpy(N, alpha, x, incx, y, incy):
+    """Host-array saxpy_hip / daxpy_hip (by y's dtype); y updated in place."""
+    L = load()
+    fn = L.daxpy_hip if y.dtype == np.float64 else L.saxpy_hip
+    fn(N, alpha, _ptr(x), incx, _ptr(y), incy)
+    return _raise_if(L.ugemm_hip_last_error())
+
+
+def gemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy):
+    """Host-array sgemv_hip / dgemv_hip (by y's dtype); y updated in place."""
+    L = load()
+    fn = L.dgemv_hip if y.dtype == np.float64 else L.sgemv_hip
+    fn(_b(trans), M, N, alpha, _ptr(A), lda, _ptr(x), incx, beta, _ptr(y), incy)
+    return _raise_if(L.ugemm_hip_last_error())
+
+
+def axpy_dev(N, alpha, dx, incx, dy, incy, double=False, stream=None):
+    s = stream.handle if hasattr(stream, "handle") else stream
+    L = load()
+    fn = L.ugemm_hip_daxpy_dev if double else L.ugemm_hip_saxpy_dev
+    return _raise_if(fn(N, alpha, _ptr(dx), incx, _ptr(dy), incy, s))
+
+
+def gemv_dev(trans, M, N, alpha, dA, lda, dx, incx, beta, dy, incy, double=False, stream=None):
+    s = stream.handle if hasattr(stream, "handle") else stream
+    L = load()
+    fn = L.ugemm_hip_dgemv_dev if double else L.ugemm_hip_sgemv_dev
+    return _raise_if(fn(_b(trans), M, N, alpha, _ptr(dA), lda, _ptr(dx), incx, beta, _ptr(dy), incy, s))
 
 
 def timing_enable(on=True):

@@ -94,6 +94,14 @@ int conv_shape(int ich, int w, int h, int k, int pad, int stride, int ch, int *h
 int launch_conv_fp32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s);
 int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t s);
 
+// ---- BLAS-1/2 helpers (blas12.hip): device pointers, stream-ordered ----
+int launch_saxpy(int N, float alpha, const float *x, int incx, float *y, int incy, hipStream_t s);
+int launch_daxpy(int N, double alpha, const double *x, int incx, double *y, int incy, hipStream_t s);
+int launch_sgemv(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,
+                 float *y, int incy, hipStream_t s);
+int launch_dgemv(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
+                 double beta, double *y, int incy, hipStream_t s);
+
 // ---- tuning (tune.hip) ----
 // Tuning-table key of grouped plans (variants = the 4 FP32 tile shapes).
 constexpr int TUNE_KEY_GROUPED = 16;

@@ -397,6 +397,100 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
     return rc ? rc : rc2;
 }
 
+static int axpy_dispatch(int N, float alpha, const float *x, int incx, float *y, int incy, hipStream_t s) {
+    return launch_saxpy(N, alpha, x, incx, y, incy, s);
+}
+static int axpy_dispatch(int N, double alpha, const double *x, int incx, double *y, int incy, hipStream_t s) {
+    return launch_daxpy(N, alpha, x, incx, y, incy, s);
+}
+static int gemv_dispatch(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx,
+                         float beta, float *y, int incy, hipStream_t s) {
+    return launch_sgemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy, s);
+}
+static int gemv_dispatch(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
+                         double beta, double *y, int incy, hipStream_t s) {
+    return launch_dgemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy, s);
+}
+
+// Host-pointer versions (the argument lists of saxpy_cpu, daxpy_cpu,
+// sgemv_cpu, dgemv_cpu; ugemm.h:76-160): stage the spans the call touches,
+// run on the internal stream, copy y back, wait.
+template <typename T>
+static void host_axpy(int N, T alpha, const T *x, int incx, T *y, int incy) {
+    int rc = ensure_device();
+    if (rc) return;
+    if (N < 0 || incx < 1 || incy < 1) {
+        fail(UGEMM_ERR_ARG, "axpy: N=%d incx=%d incy=%d", N, incx, incy);
+        return;
+    }
+    if (N == 0) {
+        ok();
+        return;
+    }
+    if (!x || !y) {
+        fail(UGEMM_ERR_ARG, "axpy: NULL vector");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    hipStream_t s = g_dev.stream;
+    const size_t nx = (size_t)(N - 1) * incx + 1, ny = (size_t)(N - 1) * incy + 1;
+    void *ws = nullptr;
+    if (hipSetDevice(g_dev.device) != hipSuccess || stream_alloc(&ws, (nx + ny) * sizeof(T) + 16, s)) return;
+    T *dx = (T *)ws, *dy = dx + nx;
+    hipError_t e = hipMemcpyAsync(dx, x, nx * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, ny * sizeof(T), hipMemcpyHostToDevice, s);
+    rc = e == hipSuccess ? axpy_dispatch(N, alpha, (const T *)dx, incx, dy, incy, s) : hip_fail(e, "axpy: copy in");
+    if (!rc && (e = hipMemcpyAsync(y, dy, ny * sizeof(T), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        rc = hip_fail(e, "axpy: copy out");
+    const int rc2 = stream_free(ws, s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess && !rc) rc = hip_fail(e, "axpy: synchronize");
+    if (!rc && !rc2) ok();
+}
+
+template <typename T>
+static void host_gemv(char trans, int M, int N, T alpha, const T *A, int lda, const T *x, int incx, T beta, T *y,
+                      int incy) {
+    int rc = ensure_device();
+    if (rc) return;
+    const bool t = !(trans == 'N' || trans == 'n');
+    if (M < 0 || N < 0 || incx < 1 || incy < 1 || lda < std::max(1, t ? N : M)) {
+        fail(UGEMM_ERR_ARG, "gemv: M=%d N=%d lda=%d incx=%d incy=%d", M, N, lda, incx, incy);
+        return;
+    }
+    if (M == 0) {
+        ok();
+        return;
+    }
+    if (!y || (N > 0 && (!A || !x))) {
+        fail(UGEMM_ERR_ARG, "gemv: NULL operand");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    hipStream_t s = g_dev.stream;
+    // A spans (cols - 1) * lda + rows elements: rows x cols column-major ('N': M x N, 'T': N x M).
+    const size_t na = N > 0 ? (size_t)((t ? M : N) - 1) * lda + (t ? N : M) : 0;
+    const size_t nx = N > 0 ? (size_t)(N - 1) * incx + 1 : 0, ny = (size_t)(M - 1) * incy + 1;
+    void *ws = nullptr;
+    if (hipSetDevice(g_dev.device) != hipSuccess || stream_alloc(&ws, (na + nx + ny) * sizeof(T) + 16, s)) return;
+    T *dA = (T *)ws, *dx = dA + na, *dy = dx + nx;
+    hipError_t e = hipSuccess;
+    if (na) e = hipMemcpyAsync(dA, A, na * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && nx) e = hipMemcpyAsync(dx, x, nx * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && beta != T(0)) e = hipMemcpyAsync(dy, y, ny * sizeof(T), hipMemcpyHostToDevice, s);
+    rc = e == hipSuccess ? gemv_dispatch(trans, M, N, alpha, (const T *)dA, lda, (const T *)dx, incx, beta, dy, incy, s)
+                         : hip_fail(e, "gemv: copy in");
+    if (!rc) {
+        // Only y's elements are written back (the gaps of a strided y keep the caller's values).
+        e = incy == 1 ? hipMemcpyAsync(y, dy, (size_t)M * sizeof(T), hipMemcpyDeviceToHost, s)
+                      : hipMemcpy2DAsync(y, incy * sizeof(T), dy, incy * sizeof(T), sizeof(T), M,
+                                         hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "gemv: copy out");
+    }
+    const int rc2 = stream_free(ws, s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess && !rc) rc = hip_fail(e, "gemv: synchronize");
+    if (!rc && !rc2) ok();
+}
+
 }  // namespace ugemm
 
 using namespace ugemm;
@@ -557,6 +651,45 @@ void convolution_LReLU_hip(float *inputs, int ich, int w, int h, float *weights,
     const int rc2 = stream_free(ws, s);
     if ((e = hipStreamSynchronize(s)) != hipSuccess && !rc) rc = hip_fail(e, "conv: synchronize");
     if (!rc && !rc2) ok();
+}
+
+// ---- BLAS-1/2 (blas12.hip) ----
+int ugemm_hip_saxpy_dev(int N, float alpha, const float *dx, int incx, float *dy, int incy, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_saxpy(N, alpha, dx, incx, dy, incy, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+int ugemm_hip_daxpy_dev(int N, double alpha, const double *dx, int incx, double *dy, int incy, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_daxpy(N, alpha, dx, incx, dy, incy, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+int ugemm_hip_sgemv_dev(char trans, int M, int N, float alpha, const float *dA, int lda, const float *dx, int incx,
+                        float beta, float *dy, int incy, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_sgemv(trans, M, N, alpha, dA, lda, dx, incx, beta, dy, incy, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+int ugemm_hip_dgemv_dev(char trans, int M, int N, double alpha, const double *dA, int lda, const double *dx,
+                        int incx, double beta, double *dy, int incy, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_dgemv(trans, M, N, alpha, dA, lda, dx, incx, beta, dy, incy, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+void saxpy_hip(int N, float alpha, const float *x, int incx, float *y, int incy) {
+    host_axpy<float>(N, alpha, x, incx, y, incy);
+}
+void daxpy_hip(int N, double alpha, const double *x, int incx, double *y, int incy) {
+    host_axpy<double>(N, alpha, x, incx, y, incy);
+}
+void sgemv_hip(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,
+               float *y, int incy) {
+    host_gemv<float>(trans, M, N, alpha, A, lda, x, incx, beta, y, incy);
+}
+void dgemv_hip(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
+               double beta, double *y, int incy) {
+    host_gemv<double>(trans, M, N, alpha, A, lda, x, incx, beta, y, incy);
 }
 
 int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int K, double alpha, const double *dA,
