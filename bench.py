@@ -21,6 +21,8 @@ Also reported (one JSON line in total):
     epilogue;
   * the precision ladder at 8192^3 with entries sign * 2^e * (1+u), e in [-12, 12];
   * the BLAS-1/2 helpers (axpy, gemv) against the HBM roofline;
+  * host-pointer calls with copies included (sgemm_test.c's workload, and
+    one 8192^3 uut call);
   * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
     into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread.
 """
@@ -392,6 +394,45 @@ def blas12(stream):
     return res
 
 
+def host_path():
+    """Host-pointer calls, copies included (the sgemm_ocl-style use of
+    sgemm_ocl2.h:166-219): sgemm_test.c's own workload (M=1023, N=1000,
+    K=1023, alpha=1, beta=0, RNN/RNT/RTN, 20 calls each, sgemm_test.c:168-172),
+    timed by the host clock as that harness does, and one 8192^3 call of the
+    uut entry sgemm_hip.  FLOP/s counts 2MNK per call."""
+    L = ug.load()
+    g = np.random.Generator(np.random.PCG64(0x55474D4D08))
+    M, N, K, times = 1023, 1000, 1023, 20
+    A = uniform(g, (M * K,))
+    B = uniform(g, (K * N,))
+    C = np.zeros(M * N, np.float32)
+    res = {}
+    for ta, tb, name in (("N", "N", "rnn"), ("N", "T", "rnt"), ("T", "N", "rtn")):
+        L.sgemm_hip_packed(ta.encode(), tb.encode(), M, N, K, 1.0, A.ctypes.data, B.ctypes.data, 0.0, C.ctypes.data)
+        t0 = time.perf_counter()
+        for _ in range(times):
+            L.sgemm_hip_packed(ta.encode(), tb.encode(), M, N, K, 1.0, A.ctypes.data, B.ctypes.data, 0.0,
+                               C.ctypes.data)
+        dt = (time.perf_counter() - t0) / times
+        if L.ugemm_hip_last_error() != 0:
+            res[name] = {"error": L.ugemm_hip_last_error_message().decode()}
+            continue
+        res[f"sgemm_test_{name}_{M}x{N}x{K}"] = {"ms_per_call": round(dt * 1e3, 4),
+                                                  "GFLOP_per_s": round(2.0 * M * N * K / dt / 1e9, 1)}
+    n = 8192
+    A = uniform(g, (n, n))
+    B = uniform(g, (n, n))
+    C = np.zeros((n, n), np.float32)
+    L.sgemm_hip(b"R", b"N", b"N", 1024, 1024, 1024, 1.0, A.ctypes.data, n, B.ctypes.data, n, 0.0, C.ctypes.data, n)
+    t0 = time.perf_counter()
+    L.sgemm_hip(b"R", b"N", b"N", n, n, n, 1.0, A.ctypes.data, n, B.ctypes.data, n, 0.0, C.ctypes.data, n)
+    dt = time.perf_counter() - t0
+    res[f"sgemm_hip_uut_{n}^3"] = {"ms_per_call": round(dt * 1e3, 3), "TFLOP_per_s": round(2.0 * n ** 3 / dt / 1e12, 2),
+                                   "host_GB_moved": round(3 * n * n * 4 / 1e9, 3)}
+    log(f"[host] {json.dumps(res)}")
+    return res
+
+
 def cpu_baseline(n, seconds_target):
     """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
     by the workload) on a row band of the n^3 workload, one thread."""
@@ -523,6 +564,7 @@ def main():
         line["config4_batched"] = config4(stream, fp32_tf)
         line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
         line["blas12"] = blas12(stream)
+        line["host_pointer_path"] = host_path()
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)

@@ -313,12 +313,15 @@ static int run_host(const Gemm<T> &h, int mode) {
     T *dA = (T *)base, *dB = (T *)(base + a_bytes), *dC = (T *)(base + a_bytes + b_bytes);
     hipStream_t s = g_dev.stream;
     const size_t es = sizeof(T);
+    // Inputs: one contiguous copy of the span an operand occupies (its last
+    // row ends at (rows - 1) * ld + len, inside the caller's buffer); the
+    // gaps between rows travel too, which costs less than a strided copy.
+    auto span = [](long rows, long ld, long len) { return (size_t)((rows - 1) * ld + len); };
     if (need_ab) {
-        UGEMM_HIP_TRY(hipMemcpy2DAsync(dA, h.lda * es, h.A, h.lda * es, a_len * es, a_rows, hipMemcpyHostToDevice, s));
-        UGEMM_HIP_TRY(hipMemcpy2DAsync(dB, h.ldb * es, h.B, h.ldb * es, b_len * es, b_rows, hipMemcpyHostToDevice, s));
+        UGEMM_HIP_TRY(hipMemcpyAsync(dA, h.A, span(a_rows, h.lda, a_len) * es, hipMemcpyHostToDevice, s));
+        UGEMM_HIP_TRY(hipMemcpyAsync(dB, h.B, span(b_rows, h.ldb, b_len) * es, hipMemcpyHostToDevice, s));
     }
-    if (need_c)
-        UGEMM_HIP_TRY(hipMemcpy2DAsync(dC, h.ldc * es, h.C, h.ldc * es, h.N * es, h.M, hipMemcpyHostToDevice, s));
+    if (need_c) UGEMM_HIP_TRY(hipMemcpyAsync(dC, h.C, span(h.M, h.ldc, h.N) * es, hipMemcpyHostToDevice, s));
     Gemm<T> d = h;
     d.A = dA;
     d.B = dB;
@@ -330,7 +333,12 @@ static int run_host(const Gemm<T> &h, int mode) {
         hipStreamSynchronize(s);
         return rc;
     }
-    UGEMM_HIP_TRY(hipMemcpy2DAsync(h.C, h.ldc * es, dC, h.ldc * es, h.N * es, h.M, hipMemcpyDeviceToHost, s));
+    // Output: contiguous when C has no gaps; otherwise row by row, so that
+    // the gaps between C's rows keep the caller's values.
+    if (h.ldc == h.N)
+        UGEMM_HIP_TRY(hipMemcpyAsync(h.C, dC, (size_t)h.M * h.N * es, hipMemcpyDeviceToHost, s));
+    else
+        UGEMM_HIP_TRY(hipMemcpy2DAsync(h.C, h.ldc * es, dC, h.ldc * es, h.N * es, h.M, hipMemcpyDeviceToHost, s));
     UGEMM_HIP_TRY(hipStreamSynchronize(s));
     return UGEMM_OK;
 }
