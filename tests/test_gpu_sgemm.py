@@ -184,3 +184,30 @@ def test_fp32_large_sampled(ta, tb):
     exact, scale = case.exact(rows=rows)
     ok, worst = orc.check_bound(C[rows, :N], exact, scale, orc.tolerance(K))
     assert ok, f"worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("mode,beta", [(ug.MODE_FP32, 0.5), (ug.MODE_FP32, 0.0), (ug.MODE_BF16X3, 0.5)])
+def test_host_pointer_pipelined_large(mode, beta):
+    """Host-pointer calls large enough for the M-block pipeline (copies of
+    block b+1 and b-1 overlap the GEMM of block b): every block of C must be
+    right, checked on rows sampled from each block."""
+    M, N, K = 4100, 4096, 520        # a short last block; 80+ MB moved
+    case = Case("R", "N", "N", M, N, K, 1.5, beta, seed=4242)
+    C = case.C0.copy()
+    ug.sgemm("R", "N", "N", M, N, K, case.alpha, case.A, case.lda, case.B, case.ldb, beta, C, case.ldc, mode=mode)
+    rows = np.unique(np.concatenate([np.arange(0, M, 512), np.arange(511, M, 512), [M - 1, M - 4, 2048, 2049]]))
+    exact, scale = case.exact(rows=rows)
+    from tests.helpers import MODE_EXTRA
+    ok, worst = orc.check_bound(C[rows, :N], exact, scale, orc.tolerance(K, extra=MODE_EXTRA.get(mode, 0.0)))
+    assert ok, worst
+
+
+def test_host_pointer_pipelined_dgemm():
+    M, N, K = 4096, 2048, 512
+    case = Case("R", "N", "N", M, N, K, 1.0, 0.25, seed=4343, dtype=np.float64)
+    C = case.C0.copy()
+    ug.dgemm("R", "N", "N", M, N, K, 1.0, case.A, case.lda, case.B, case.ldb, 0.25, C, case.ldc)
+    rows = np.array([0, 1, 1023, 1024, 2047, 2048, 3071, 4095])
+    exact, scale = case.exact(rows=rows)
+    ok, worst = orc.check_bound(C[rows, :N], exact, scale, orc.tolerance(K, eps=orc.EPS64))
+    assert ok, worst

@@ -116,6 +116,15 @@ void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, f
 void timing_begin(hipStream_t s);
 void timing_end(hipStream_t s);
 
+// ---- pinned staging for host-pointer calls (staging.hip) ----
+// Pageable host memory <-> device through two pinned slots, chunked so that
+// host-side copies (split over a few threads) overlap the DMA.  Stream-ordered
+// on s; staged_d2h returns when the data is in dst.  Not thread-safe: callers
+// hold the device mutex.
+int staged_h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+int staged_d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+void release_staging();
+
 // ---- device scratch (runtime.hip) ----
 // Stream-ordered workspace for the operand-preparation kernels: allocated
 // from the device's default memory pool on stream s (which keeps freed

@@ -51,7 +51,8 @@ struct DeviceState {
     bool ready = false;
     int device = 0;
     int cus = 256;
-    hipStream_t stream = nullptr;   // internal stream for host-pointer calls
+    hipStream_t stream = nullptr;   // internal stream for host-pointer calls (copies)
+    hipStream_t kstream = nullptr;  // second internal stream: compute of pipelined host-pointer calls
     void *staging = nullptr;        // device staging for host-pointer calls
     size_t staging_bytes = 0;
 };
@@ -78,6 +79,7 @@ static int init_locked(int device, size_t max_bytes) {
         g_dev.device = device;
         g_dev.cus = prop.multiProcessorCount;
         UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.kstream, hipStreamNonBlocking));
         // Keep freed workspace blocks in the default pool (no release at
         // synchronisation points), so that repeated calls do not go back to
         // the driver for memory.
@@ -317,16 +319,83 @@ static int run_host(const Gemm<T> &h, int mode) {
     // row ends at (rows - 1) * ld + len, inside the caller's buffer); the
     // gaps between rows travel too, which costs less than a strided copy.
     auto span = [](long rows, long ld, long len) { return (size_t)((rows - 1) * ld + len); };
-    if (need_ab) {
-        UGEMM_HIP_TRY(hipMemcpyAsync(dA, h.A, span(a_rows, h.lda, a_len) * es, hipMemcpyHostToDevice, s));
-        UGEMM_HIP_TRY(hipMemcpyAsync(dB, h.B, span(b_rows, h.ldb, b_len) * es, hipMemcpyHostToDevice, s));
+    int rc = UGEMM_OK;
+    // Large calls whose A rows and C rows are contiguous blocks: split along
+    // M so that block b's GEMM (second stream) runs while block b+1's rows of
+    // A come in and block b-1's rows of C go out (copy stream).
+    if (!h.ta && h.ldc == h.N && need_ab && h.M >= 2048 && total >= (64u << 20)) {
+        hipStream_t ks = g_dev.kstream;
+        const int nb = (int)std::min<long>(8, std::max<long>(2, h.M / 2048));
+        const long Mb = (long)round_up((size_t)((h.M + nb - 1) / nb), 256);
+        std::vector<hipEvent_t> evA, evC;
+        auto cleanup = [&]() {
+            hipStreamSynchronize(ks);
+            hipStreamSynchronize(s);
+            for (hipEvent_t e : evA) hipEventDestroy(e);
+            for (hipEvent_t e : evC) hipEventDestroy(e);
+        };
+        rc = staged_h2d(dB, h.B, span(b_rows, h.ldb, b_len) * es, s);
+        long prev0 = -1, prev1 = -1;
+        for (long m0 = 0; !rc && m0 < h.M; m0 += Mb) {
+            const long m1 = std::min<long>(h.M, m0 + Mb);
+            hipEvent_t ea = nullptr, ec = nullptr;
+            if (hipEventCreateWithFlags(&ea, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&ec, hipEventDisableTiming) != hipSuccess) {
+                if (ea) hipEventDestroy(ea);
+                rc = fail(UGEMM_ERR_HIP, "pipelined copy: events");
+                break;
+            }
+            evA.push_back(ea);
+            evC.push_back(ec);
+            rc = staged_h2d(dA + m0 * h.lda, h.A + m0 * h.lda, span(m1 - m0, h.lda, a_len) * es, s);
+            if (!rc && need_c) rc = staged_h2d(dC + m0 * h.N, h.C + m0 * h.N, (size_t)(m1 - m0) * h.N * es, s);
+            if (rc) break;
+            if (hipEventRecord(ea, s) != hipSuccess || hipStreamWaitEvent(ks, ea, 0) != hipSuccess) {
+                rc = fail(UGEMM_ERR_HIP, "pipelined copy: ordering");
+                break;
+            }
+            Gemm<T> d = h;
+            d.M = (int)(m1 - m0);
+            d.A = dA + m0 * h.lda;
+            d.B = dB;
+            d.C = dC + m0 * h.N;
+            if constexpr (sizeof(T) == 4) rc = run_sgemm(d, mode, -1, ks);
+            else rc = run_dgemm(d, -1, ks);
+            if (rc) break;
+            if (hipEventRecord(ec, ks) != hipSuccess) {
+                rc = fail(UGEMM_ERR_HIP, "pipelined copy: ordering");
+                break;
+            }
+            if (prev0 >= 0) {
+                if (hipStreamWaitEvent(s, evC[evC.size() - 2], 0) != hipSuccess) {
+                    rc = fail(UGEMM_ERR_HIP, "pipelined copy: ordering");
+                    break;
+                }
+                rc = staged_d2h(h.C + prev0 * h.N, dC + prev0 * h.N, (size_t)(prev1 - prev0) * h.N * es, s);
+            }
+            prev0 = m0;
+            prev1 = m1;
+        }
+        if (!rc && prev0 >= 0) {
+            if (hipStreamWaitEvent(s, evC.back(), 0) != hipSuccess) rc = fail(UGEMM_ERR_HIP, "pipelined copy: ordering");
+            else rc = staged_d2h(h.C + prev0 * h.N, dC + prev0 * h.N, (size_t)(prev1 - prev0) * h.N * es, s);
+        }
+        cleanup();
+        return rc;
     }
-    if (need_c) UGEMM_HIP_TRY(hipMemcpyAsync(dC, h.C, span(h.M, h.ldc, h.N) * es, hipMemcpyHostToDevice, s));
+    if (need_ab) {
+        rc = staged_h2d(dA, h.A, span(a_rows, h.lda, a_len) * es, s);
+        if (!rc) rc = staged_h2d(dB, h.B, span(b_rows, h.ldb, b_len) * es, s);
+    }
+    if (!rc && need_c) rc = staged_h2d(dC, h.C, span(h.M, h.ldc, h.N) * es, s);
+    if (rc) {
+        hipStreamSynchronize(s);
+        return rc;
+    }
     Gemm<T> d = h;
     d.A = dA;
     d.B = dB;
     d.C = dC;
-    int rc;
     if constexpr (sizeof(T) == 4) rc = run_sgemm(d, mode, -1, s);
     else rc = run_dgemm(d, -1, s);
     if (rc) {
@@ -335,12 +404,15 @@ static int run_host(const Gemm<T> &h, int mode) {
     }
     // Output: contiguous when C has no gaps; otherwise row by row, so that
     // the gaps between C's rows keep the caller's values.
-    if (h.ldc == h.N)
-        UGEMM_HIP_TRY(hipMemcpyAsync(h.C, dC, (size_t)h.M * h.N * es, hipMemcpyDeviceToHost, s));
-    else
-        UGEMM_HIP_TRY(hipMemcpy2DAsync(h.C, h.ldc * es, dC, h.ldc * es, h.N * es, h.M, hipMemcpyDeviceToHost, s));
-    UGEMM_HIP_TRY(hipStreamSynchronize(s));
-    return UGEMM_OK;
+    if (h.ldc == h.N) {
+        rc = staged_d2h(h.C, dC, (size_t)h.M * h.N * es, s);
+    } else {
+        hipError_t e = hipMemcpy2DAsync(h.C, h.ldc * es, dC, h.ldc * es, h.N * es, h.M, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy out");
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "synchronize");
+    return rc;
 }
 
 static int env_mode() {
@@ -536,8 +608,14 @@ void ugemm_hip_finish(void) {
     if (!g_dev.ready) return;
     hipSetDevice(g_dev.device);
     if (g_dev.stream) hipStreamSynchronize(g_dev.stream);
+    release_staging();
     if (g_dev.staging) hipFree(g_dev.staging);
     if (g_dev.stream) hipStreamDestroy(g_dev.stream);
+    if (g_dev.kstream) {
+        hipStreamSynchronize(g_dev.kstream);
+        hipStreamDestroy(g_dev.kstream);
+    }
+    g_dev.kstream = nullptr;
     g_dev.staging = nullptr;
     g_dev.staging_bytes = 0;
     g_dev.stream = nullptr;
