@@ -354,6 +354,10 @@ struct CoreCfg {
     static constexpr int STAGES = (2 * STAGE * 2 <= 160 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = STAGES * STAGE * 2;
     static constexpr int CHUNKS = NP * (BM + BN) * CPR;                 // 16-byte chunks per stage
+    // 8-wave variants can also split a k-contiguous FP32 operand while
+    // staging it (no preparation pass for that operand); 16 waves have no
+    // registers to spare for it.
+    static constexpr bool FUSABLE = 64 * WGM_ * WGN_ == 512;
     static constexpr int PER_T = CHUNKS / NT;
     static_assert(CHUNKS % NT == 0, "stage must split evenly over the threads");
     static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 32 == 0, "tile shape");
@@ -361,13 +365,21 @@ struct CoreCfg {
 
 // amax_a / amax_b: per-row largest magnitudes of the scaled formats
 // (FMT::SCALED), unused otherwise.
-template <class CF>
+// FA / FB: that operand is split while staging, from its k-contiguous FP32
+// source Af[m*lda + k] / Bf[n*ldb + k] (K columns, zeros beyond M, N, K),
+// instead of being copied from prepared planes.  Thread t handles the chunks
+// f = t + NT*i of the stage (row r = f / 4, stored chunk f % 4, which holds
+// k-chunk (f % 4) ^ sw(r)), so each split chunk lands where the plane copy
+// would have put it.
+template <class CF, bool FA, bool FB>
 __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, const typename CF::FMT::E *__restrict__ Ap,
                                                        long a_plane, int Mp, const typename CF::FMT::E *__restrict__ Bp,
                                                        long b_plane, int Np, float alpha, float beta,
                                                        float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
                                                        const unsigned *__restrict__ amax_a,
-                                                       const unsigned *__restrict__ amax_b, Epilogue epi) {
+                                                       const unsigned *__restrict__ amax_b, Epilogue epi,
+                                                       const float *__restrict__ Af, long lda,
+                                                       const float *__restrict__ Bf, long ldb, int K, int vec_ab) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
@@ -392,24 +404,94 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                   "stage split");
     const E *a_src = Ap + (long)m0 * BK + t * 8;
     const E *b_src = Bp + (long)n0 * BK + t * 8;
-    V8 stage_regs[PER_T];
-    auto load_stage = [&](int kt) {
-        const long a_k = (long)kt * Mp * BK, b_k = (long)kt * Np * BK;
+    constexpr int B_CH = NP * BN * CPR / CF::NT;          // 16-byte chunks per thread from B planes
+    constexpr int FA_CH = BM * CPR / CF::NT, FB_CH = BN * CPR / CF::NT;   // fused chunk rows per thread
+    static_assert(CPR == 4, "fused staging assumes 32-wide K tiles");
+    V8 pa[FA ? 1 : A_CH], pb[FB ? 1 : B_CH];
+    float fa[FA ? FA_CH : 1][8], fb[FB ? FB_CH : 1][8];
+    // 8 floats of row `row`, k in [k, k+8), zeros outside rows x K.
+    auto load8 = [&](const float *X, long ld, int rows, int row, int k, bool vec, float (&v)[8]) {
+        const float *p = X + (long)row * ld + k;
+        if (row < rows && k + 8 <= K && vec) {
+            const float4 x0 = *reinterpret_cast<const float4 *>(p);
+            const float4 x1 = *reinterpret_cast<const float4 *>(p + 4);
+            v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w; v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+        } else {
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) {
-            if (i < A_CH) {
+            for (int j = 0; j < 8; ++j) v[j] = (row < rows && k + j < K) ? p[j] : 0.0f;
+        }
+    };
+    auto load_stage = [&](int kt) {
+        const int k0 = kt * BK;
+        if constexpr (FA) {
+#pragma unroll
+            for (int i = 0; i < FA_CH; ++i) {
+                const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
+                load8(Af, lda, M, m0 + r, k0 + c * 8, vec_ab & 1, fa[i]);
+            }
+        } else {
+            const long a_k = (long)kt * Mp * BK;
+#pragma unroll
+            for (int i = 0; i < A_CH; ++i) {
                 const int e = i * CF::NT * 8, p = e / (BM * BK), off = e % (BM * BK);
-                stage_regs[i] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + a_k + off);
-            } else {
-                const int e = (i - A_CH) * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
-                stage_regs[i] = *reinterpret_cast<const V8 *>(b_src + p * b_plane + b_k + off);
+                pa[i] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + a_k + off);
+            }
+        }
+        if constexpr (FB) {
+#pragma unroll
+            for (int i = 0; i < FB_CH; ++i) {
+                const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
+                load8(Bf, ldb, N, n0 + r, k0 + c * 8, (vec_ab & 2) != 0, fb[i]);
+            }
+        } else {
+            const long b_k = (long)kt * Np * BK;
+#pragma unroll
+            for (int i = 0; i < B_CH; ++i) {
+                const int e = i * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
+                pb[i] = *reinterpret_cast<const V8 *>(b_src + p * b_plane + b_k + off);
             }
         }
     };
-    auto store_stage = [&](int buf) {
-        E *base = lds + buf * STAGE + t * 8;
+    // Split one fused chunk (scaled formats: times 2^-e of its row) into its
+    // NP plane chunks at plane offset f * 8 of the operand's LDS region.
+    auto store_fused = [&](E *region, int plane, const unsigned *amax, int row, int f, float (&v)[8]) {
+        if constexpr (FMT::SCALED) {
+            const int e = -row_exp(amax[row]);
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<V8 *>(base + i * CF::NT * 8) = stage_regs[i];
+            for (int j = 0; j < 8; ++j) v[j] = ldexpf(v[j], e);
+        }
+        E o[NP][8];
+        split_vec<NP, 8>(v, o);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            V8 w;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = o[p][j];
+            *reinterpret_cast<V8 *>(region + p * plane + f * 8) = w;
+        }
+    };
+    auto store_stage = [&](int buf) {
+        E *a_reg = lds + buf * STAGE, *b_reg = a_reg + NP * CF::A_PLANE;
+        if constexpr (FA) {
+#pragma unroll
+            for (int i = 0; i < FA_CH; ++i) {
+                const int f = t + CF::NT * i;
+                store_fused(a_reg, CF::A_PLANE, amax_a, m0 + (f >> 2), f, fa[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < A_CH; ++i) *reinterpret_cast<V8 *>(a_reg + t * 8 + i * CF::NT * 8) = pa[i];
+        }
+        if constexpr (FB) {
+#pragma unroll
+            for (int i = 0; i < FB_CH; ++i) {
+                const int f = t + CF::NT * i;
+                store_fused(b_reg, CF::B_PLANE, amax_b, n0 + (f >> 2), f, fb[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < B_CH; ++i) *reinterpret_cast<V8 *>(b_reg + t * 8 + i * CF::NT * 8) = pb[i];
+        }
     };
 
     floatx4 acc[FM][FN];
@@ -612,9 +694,11 @@ static long round_up(long x, long a) { return (x + a - 1) / a * a; }
 
 // Scaled formats: amax (rows_pad entries, zeroed by the caller) receives the
 // per-row largest magnitudes before the split reads them.
+// split == false: only the row maxima (scaled formats); the core splits the
+// operand itself while staging it.
 template <int NP, class FMT>
 int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename FMT::E *out, long plane_stride,
-                 int rows_pad, int k_pad, unsigned *amax, hipStream_t s) {
+                 int rows_pad, int k_pad, unsigned *amax, hipStream_t s, bool split = true) {
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
     if constexpr (FMT::SCALED) {
         if (rows > 0 && K > 0) {
@@ -628,6 +712,10 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
                                    K, vec, kblocks, amax);
             }
         }
+    }
+    if (!split) {
+        UGEMM_HIP_TRY(hipGetLastError());
+        return UGEMM_OK;
     }
     if (kcont) {
         const long total = (long)rows_pad * (k_pad / 8);
@@ -644,48 +732,69 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
     return UGEMM_OK;
 }
 
-// Workspace: NP planes of A (Mp x Kp) and of B (Np x Kp), then (scaled
-// formats) Mp + Np row maxima.
+template <class CF, bool FA, bool FB>
+int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plane, int Mp, const typename CF::FMT::E *Bp,
+                long b_plane, int Np, int Kp, int tiles_m, int tiles_n, const unsigned *amax_a,
+                const unsigned *amax_b, hipStream_t s) {
+    auto kern = split_nt_core<CF, FA, FB>;
+    // Once per kernel instantiation (thread-safe static initialisation).
+    static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
+    if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
+    const int vec_ab = ((ugemm_aligned16(g.A) && g.lda % 4 == 0) ? 1 : 0) | ((ugemm_aligned16(g.B) && g.ldb % 4 == 0) ? 2 : 0);
+    timing_begin(s);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N, Kp, Ap,
+                       a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n, amax_a, amax_b,
+                       g.epi, g.A, g.lda, g.B, g.ldb, g.K, vec_ab);
+    timing_end(s);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UGEMM_OK : hip_fail(e, "split core launch");
+}
+
+// Workspace: NP planes of A (Mp x Kp) and of B (Np x Kp) -- none for an
+// operand the core splits itself -- then (scaled formats) Mp + Np row maxima.
 template <class CF>
 int run_cfg(const Gemm<float> &g, hipStream_t s) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     constexpr int NP = CF::NP;
     const long Mp = round_up(g.M, CF::BM), Np = round_up(g.N, CF::BN), Kp = round_up(g.K, 64);
+    // A k-contiguous FP32 source (A not transposed, B transposed) read by at
+    // most two tiles along the other dimension is split in the core (8-wave
+    // variants): its preparation pass would cost as much as the GEMM reads.
+    // Operands re-read by many tiles stay on prepared planes, whose blocked
+    // layout the core streams contiguously (measured: splitting a re-read
+    // operand in the core is ~50% slower at 8192^3).
+    const bool fa = CF::FUSABLE && !g.ta && Np / CF::BN <= 2, fb = CF::FUSABLE && g.tb && Mp / CF::BM <= 2;
     const long a_plane = Mp * Kp, b_plane = Np * Kp;
-    const size_t plane_bytes = (size_t)NP * (a_plane + b_plane) * sizeof(E);
+    const size_t a_bytes = fa ? 0 : (size_t)NP * a_plane * sizeof(E), b_bytes = fb ? 0 : (size_t)NP * b_plane * sizeof(E);
     const size_t amax_bytes = FMT::SCALED ? (size_t)(Mp + Np) * sizeof(unsigned) : 0;
     const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
     if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
     if (FMT::SCALED && (long)g.M * ((g.K + 2047) / 2048) > 0x7fffffffL)
         return fail(UGEMM_ERR_ARG, "too many row blocks");
     void *ws = nullptr;
-    int rc = stream_alloc(&ws, plane_bytes + amax_bytes, s);
+    int rc = stream_alloc(&ws, a_bytes + b_bytes + amax_bytes + 256, s);
     if (rc) return rc;
-    E *Ap = (E *)ws, *Bp = Ap + NP * a_plane;
-    unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + plane_bytes) : nullptr;
+    E *Ap = (E *)ws, *Bp = (E *)((char *)ws + a_bytes);
+    unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + a_bytes + b_bytes) : nullptr;
     unsigned *amax_b = FMT::SCALED ? amax_a + Mp : nullptr;
     if (FMT::SCALED) {
         // A fill kernel rather than a memset: callers may capture this call in a graph.
         hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
                            s, amax_a, Mp + Np);
     }
-    rc = prep_operand<NP, FMT>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s);
-    if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s);
+    rc = prep_operand<NP, FMT>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s, !fa);
+    if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s, !fb);
     if (!rc) {
-        auto kern = split_nt_core<CF>;
-        // Once per kernel instantiation (thread-safe static initialisation).
-        static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
-        if (attr != hipSuccess) rc = hip_fail(attr, "hipFuncSetAttribute");
-        if (!rc) {
-            timing_begin(s);
-            hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N,
-                               (int)Kp, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, g.alpha, g.beta, g.C, g.ldc, tiles_m,
-                               tiles_n, (const unsigned *)amax_a, (const unsigned *)amax_b, g.epi);
-            timing_end(s);
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) rc = hip_fail(e, "split core launch");
+        const unsigned *ma = amax_a, *mb = amax_b;
+        if constexpr (CF::FUSABLE) {
+            if (fa && fb) rc = launch_core<CF, true, true>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+            else if (fa) rc = launch_core<CF, true, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+            else if (fb) rc = launch_core<CF, false, true>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+            else rc = launch_core<CF, false, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+        } else {
+            rc = launch_core<CF, false, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
         }
     }
     int rc2 = stream_free(ws, s);
@@ -742,7 +851,13 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
     // multi-wave variant measured fastest for the form.
     const long big_m = 256, big_n = (np == 3) ? 128 : 256;
     const long tiles_big = ((g.M + big_m - 1) / big_m) * ((g.N + big_n - 1) / big_n);
-    if (tiles_big >= cus) return 5;
+    if (tiles_big >= cus) {
+        // A skinny side whose operand the 8-wave variant splits in the core
+        // (no preparation pass): V4; otherwise the 16-wave V5.
+        const long tm = (g.M + big_m - 1) / big_m, tn = (g.N + big_n - 1) / big_n;
+        const bool fuse = np <= 2 && ((!g.ta && tn <= 2) || (g.tb && tm <= 2));
+        return fuse ? 4 : 5;
+    }
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= cus / 2) return 1;
     return 3;
