@@ -30,7 +30,7 @@ struct DCfg {
 };
 
 template <class CF, bool TA, bool TB, int VEC>
-__global__ __launch_bounds__(CF::NT) void dgemm_mfma_kernel(int M, int N, int K, double alpha,
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_mfma_kernel(int M, int N, int K, double alpha,
                                                             const double *__restrict__ A, long lda,
                                                             const double *__restrict__ B, long ldb, double beta,
                                                             double *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
