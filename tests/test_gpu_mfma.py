@@ -59,6 +59,27 @@ def test_split_every_variant(mode, variant, ta, tb):
         ug.force_variant(mode, -1)
 
 
+@pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
+@pytest.mark.parametrize("variant", [4, 6])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("pad", [0, 1])
+def test_split_operand_fused_in_core(mode, variant, ta, tb, pad):
+    """The 8-wave variants split an operand that at most two tiles re-read
+    while staging it, from either FP32 layout: k-contiguous (A in N, B in T)
+    or row-contiguous (A in T, B in N), the latter transposed in registers.
+    One side of each shape has at most 2 tiles of 256, the other 5; odd row
+    counts and pad = 1 (odd leading dimensions: scalar loads) hit the edges."""
+    ug.force_variant(mode, variant)
+    try:
+        for (M, N, K) in ((301, 1100, 77), (1100, 257, 96), (3, 1031, 40), (1030, 511, 33)):
+            case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=M * 3 + N + K + pad, pad=pad)
+            C = run_host(case, mode)
+            ok, worst = case.judge(C, mode=mode)
+            assert ok, f"variant {variant} {ta}{tb} {M}x{N}x{K} pad {pad}: {worst:.3g}"
+    finally:
+        ug.force_variant(mode, -1)
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("pad", [1, 4])
 def test_split_leading_dimensions_and_nan_c(mode, pad):

@@ -372,15 +372,22 @@ struct CoreCfg {
     static_assert(!ILV || (PHASED && STAGES == 2 && KSTEPS == 1), "interleaved staging needs phased reads, two stages");
 };
 
+enum { FUSE_NONE = 0, FUSE_KCONT = 1, FUSE_RCONT = 2 };
+
 // amax_a / amax_b: per-row largest magnitudes of the scaled formats
 // (FMT::SCALED), unused otherwise.
-// FA / FB: that operand is split while staging, from its k-contiguous FP32
-// source Af[m*lda + k] / Bf[n*ldb + k] (K columns, zeros beyond M, N, K),
-// instead of being copied from prepared planes.  Thread t handles the chunks
-// f = t + NT*i of the stage (row r = f / 4, stored chunk f % 4, which holds
-// k-chunk (f % 4) ^ sw(r)), so each split chunk lands where the plane copy
-// would have put it.
-template <class CF, bool FA, bool FB>
+// FA / FB (FUSE_*): where the core takes that operand's stage from.
+//   FUSE_NONE:  prepared planes, copied verbatim.
+//   FUSE_KCONT: split while staging from the k-contiguous FP32 source
+//     X[r*ld + k].  Thread t handles the chunks f = t + NT*i of the stage
+//     (row r = f / 4, stored chunk f % 4, which holds k-chunk (f % 4) ^ sw(r)).
+//   FUSE_RCONT: split while staging from the row-contiguous FP32 source
+//     X[k*ld + r], transposed in registers: thread t loads 8 k-lines of the
+//     RPT = 2 adjacent rows rp*2, rp*2 + 1 (rp = t % (rows/2), k-chunk
+//     t / (rows/2)), coalesced along r, and splits one 8-k chunk per row.
+// Either way each split chunk lands where the plane copy would have put it
+// (zeros beyond M, N, K).
+template <class CF, int FA, int FB>
 __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, const typename CF::FMT::E *__restrict__ Ap,
                                                        long a_plane, int Mp, const typename CF::FMT::E *__restrict__ Bp,
                                                        long b_plane, int Np, float alpha, float beta,
@@ -416,8 +423,26 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     constexpr int B_CH = NP * BN * CPR / CF::NT;          // 16-byte chunks per thread from B planes
     constexpr int FA_CH = BM * CPR / CF::NT, FB_CH = BN * CPR / CF::NT;   // fused chunk rows per thread
     static_assert(CPR == 4, "fused staging assumes 32-wide K tiles");
+    static_assert((FA != FUSE_RCONT || FA_CH == 2) && (FB != FUSE_RCONT || FB_CH == 2),
+                  "row-contiguous fusion: two rows per thread");
     V8 pa[FA ? 1 : A_CH], pb[FB ? 1 : B_CH];
     float fa[FA ? FA_CH : 1][8], fb[FB ? FB_CH : 1][8];
+    // Row-contiguous source: rows [r, r+2) of k-lines k .. k+7 into v[i][j]
+    // (row r+i, k+j); zeros outside rows x K.  vec: 8-byte loads allowed.
+    auto load_rc = [&](const float *X, long ld, int rows, int r, int k, bool vec, float (&v)[2][8]) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float *p = X + (long)(k + j) * ld + r;
+            if (k + j < K && r + 2 <= rows && vec) {
+                const float2 x = *reinterpret_cast<const float2 *>(p);
+                v[0][j] = x.x;
+                v[1][j] = x.y;
+            } else {
+                v[0][j] = (k + j < K && r < rows) ? p[0] : 0.0f;
+                v[1][j] = (k + j < K && r + 1 < rows) ? p[1] : 0.0f;
+            }
+        }
+    };
     // 8 floats of row `row`, k in [k, k+8), zeros outside rows x K.
     auto load8 = [&](const float *X, long ld, int rows, int row, int k, bool vec, float (&v)[8]) {
         const float *p = X + (long)row * ld + k;
@@ -432,7 +457,9 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     };
     auto load_stage = [&](int kt) {
         const int k0 = kt * BK;
-        if constexpr (FA) {
+        if constexpr (FA == FUSE_RCONT) {
+            load_rc(Af, lda, M, m0 + (t % (BM / 2)) * 2, k0 + (t / (BM / 2)) * 8, vec_ab & 1, fa);
+        } else if constexpr (FA == FUSE_KCONT) {
 #pragma unroll
             for (int i = 0; i < FA_CH; ++i) {
                 const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
@@ -446,7 +473,9 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                 pa[i] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + a_k + off);
             }
         }
-        if constexpr (FB) {
+        if constexpr (FB == FUSE_RCONT) {
+            load_rc(Bf, ldb, N, n0 + (t % (BN / 2)) * 2, k0 + (t / (BN / 2)) * 8, (vec_ab & 2) != 0, fb);
+        } else if constexpr (FB == FUSE_KCONT) {
 #pragma unroll
             for (int i = 0; i < FB_CH; ++i) {
                 const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
@@ -481,7 +510,13 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     };
     auto store_stage = [&](int buf) {
         E *a_reg = lds + buf * STAGE, *b_reg = a_reg + NP * CF::A_PLANE;
-        if constexpr (FA) {
+        if constexpr (FA == FUSE_RCONT) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = (t % (BM / 2)) * 2 + i, f = r * 4 + ((t / (BM / 2)) ^ ((r >> 1) & 3));
+                store_fused(a_reg, CF::A_PLANE, amax_a, m0 + r, f, fa[i]);
+            }
+        } else if constexpr (FA == FUSE_KCONT) {
 #pragma unroll
             for (int i = 0; i < FA_CH; ++i) {
                 const int f = t + CF::NT * i;
@@ -491,7 +526,13 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
 #pragma unroll
             for (int i = 0; i < A_CH; ++i) *reinterpret_cast<V8 *>(a_reg + t * 8 + i * CF::NT * 8) = pa[i];
         }
-        if constexpr (FB) {
+        if constexpr (FB == FUSE_RCONT) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = (t % (BN / 2)) * 2 + i, f = r * 4 + ((t / (BN / 2)) ^ ((r >> 1) & 3));
+                store_fused(b_reg, CF::B_PLANE, amax_b, n0 + r, f, fb[i]);
+            }
+        } else if constexpr (FB == FUSE_KCONT) {
 #pragma unroll
             for (int i = 0; i < FB_CH; ++i) {
                 const int f = t + CF::NT * i;
@@ -768,7 +809,7 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
     return UGEMM_OK;
 }
 
-template <class CF, bool FA, bool FB>
+template <class CF, int FA, int FB>
 int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plane, int Mp, const typename CF::FMT::E *Bp,
                 long b_plane, int Np, int Kp, int tiles_m, int tiles_n, const unsigned *amax_a,
                 const unsigned *amax_b, hipStream_t s) {
@@ -777,7 +818,12 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
     static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
     if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
-    const int vec_ab = ((ugemm_aligned16(g.A) && g.lda % 4 == 0) ? 1 : 0) | ((ugemm_aligned16(g.B) && g.ldb % 4 == 0) ? 2 : 0);
+    // Vector loads of a fused operand: 16 bytes (k-contiguous) or 8 bytes (row-contiguous).
+    auto vec_ok = [](const float *X, long ld, int fuse) {
+        return fuse == FUSE_RCONT ? ((reinterpret_cast<uintptr_t>(X) & 7) == 0 && ld % 2 == 0)
+                                  : (ugemm_aligned16(X) && ld % 4 == 0);
+    };
+    const int vec_ab = (vec_ok(g.A, g.lda, FA) ? 1 : 0) | (vec_ok(g.B, g.ldb, FB) ? 2 : 0);
     timing_begin(s);
     hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N, Kp, Ap,
                        a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n, amax_a, amax_b,
@@ -801,7 +847,15 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     // Operands re-read by many tiles stay on prepared planes, whose blocked
     // layout the core streams contiguously (measured: splitting a re-read
     // operand in the core is ~50% slower at 8192^3).
-    const bool fa = CF::FUSABLE && !g.ta && Np / CF::BN <= 2, fb = CF::FUSABLE && g.tb && Mp / CF::BM <= 2;
+    // A row-contiguous source (A transposed, B not) is transposed in
+    // registers on the way; that needs two rows per thread.  At most one
+    // operand is fused per call (the larger, when both qualify).
+    constexpr bool RC_A = CF::BM * CF::CPR / CF::NT == 2, RC_B = CF::BN * CF::CPR / CF::NT == 2;
+    bool fa = CF::FUSABLE && Np / CF::BN <= 2 && (!g.ta || RC_A);
+    bool fb = CF::FUSABLE && Mp / CF::BM <= 2 && (g.tb || RC_B);
+    if (fa && fb) (g.M >= g.N ? fb : fa) = false;
+    const int fa_mode = !fa ? FUSE_NONE : (g.ta ? FUSE_RCONT : FUSE_KCONT);
+    const int fb_mode = !fb ? FUSE_NONE : (g.tb ? FUSE_KCONT : FUSE_RCONT);
     const long a_plane = Mp * Kp, b_plane = Np * Kp;
     const size_t a_bytes = fa ? 0 : (size_t)NP * a_plane * sizeof(E), b_bytes = fb ? 0 : (size_t)NP * b_plane * sizeof(E);
     const size_t amax_bytes = FMT::SCALED ? (size_t)(Mp + Np) * sizeof(unsigned) : 0;
@@ -824,14 +878,20 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s, !fb);
     if (!rc) {
         const unsigned *ma = amax_a, *mb = amax_b;
+#define UGEMM_CORE(FA_, FB_) \
+    launch_core<CF, FA_, FB_>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s)
         if constexpr (CF::FUSABLE) {
-            if (fa && fb) rc = launch_core<CF, true, true>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
-            else if (fa) rc = launch_core<CF, true, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
-            else if (fb) rc = launch_core<CF, false, true>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
-            else rc = launch_core<CF, false, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+            if (fa_mode == FUSE_KCONT) rc = UGEMM_CORE(FUSE_KCONT, FUSE_NONE);
+            else if (fb_mode == FUSE_KCONT) rc = UGEMM_CORE(FUSE_NONE, FUSE_KCONT);
+            else if (fa_mode == FUSE_RCONT) {
+                if constexpr (RC_A) rc = UGEMM_CORE(FUSE_RCONT, FUSE_NONE);
+            } else if (fb_mode == FUSE_RCONT) {
+                if constexpr (RC_B) rc = UGEMM_CORE(FUSE_NONE, FUSE_RCONT);
+            } else rc = UGEMM_CORE(FUSE_NONE, FUSE_NONE);
         } else {
-            rc = launch_core<CF, false, false>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s);
+            rc = UGEMM_CORE(FUSE_NONE, FUSE_NONE);
         }
+#undef UGEMM_CORE
     }
     int rc2 = stream_free(ws, s);
     return rc ? rc : rc2;
@@ -890,12 +950,12 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
     const long big_m = 256, big_n = (np == 3) ? 128 : 256;
     const long tiles_big = ((g.M + big_m - 1) / big_m) * ((g.N + big_n - 1) / big_n);
     if (tiles_big >= cus) {
-        // A skinny side whose operand the 8-wave variant splits in the core
-        // (no preparation pass): V4.  Otherwise two planes run fastest on
+        // A skinny side whose operand (either layout) the 8-wave variant
+        // splits in the core (no preparation pass): V4.  Otherwise two planes run fastest on
         // the 8-wave interleaved-staging V6, one plane and three planes on
         // the 16-wave V5.
         const long tm = (g.M + big_m - 1) / big_m, tn = (g.N + big_n - 1) / big_n;
-        const bool fuse = np <= 2 && ((!g.ta && tn <= 2) || (g.tb && tm <= 2));
+        const bool fuse = np <= 2 && (tn <= 2 || tm <= 2);
         return fuse ? 4 : (np == 2 ? 6 : 5);
     }
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
