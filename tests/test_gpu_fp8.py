@@ -49,15 +49,20 @@ def test_fp8_matches_emulation(major, ta, tb):
         assert ok, f"{major}{ta}{tb} {M}x{N}x{K}: worst/bound {worst:.3g}"
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_fp8_variants_and_wide_range(variant):
+    """Every compiled tile variant against the emulation, including the
+    8-wave interleaved one at 1, 2 and 5 K tiles (its pipeline's edges)."""
+    assert len(ug.variants(ug.MODE_FP8_BLOCK)) == 3
     ug.force_variant(ug.MODE_FP8_BLOCK, variant)
     try:
-        case = Case("R", "N", "N", 300, 280, 520, 1.0, 0.0, seed=17 + variant, dist="wide")
-        C = run(case)
-        exact, scale = emulate(case)
-        ok, worst = orc.check_bound(case.logical(C), exact, scale, orc.tolerance(case.K, extra=FP8_EXTRA))
-        assert ok, worst
+        for (M, N, K, beta, ta, tb) in ((300, 280, 520, 0.0, "N", "N"), (600, 530, 128, 0.5, "N", "T"),
+                                        (257, 513, 256, 0.5, "T", "N"), (520, 300, 700, 1.0, "T", "T")):
+            case = Case("R", ta, tb, M, N, K, 1.0, beta, seed=17 + variant + M, dist="wide")
+            C = run(case)
+            exact, scale = emulate(case)
+            ok, worst = orc.check_bound(case.logical(C), exact, scale, orc.tolerance(case.K, extra=FP8_EXTRA))
+            assert ok, f"variant {variant} {ta}{tb} {M}x{N}x{K}: {worst:.3g}"
     finally:
         ug.force_variant(ug.MODE_FP8_BLOCK, -1)
 

@@ -27,6 +27,7 @@ namespace ugemm {
 namespace {
 
 typedef int int8v __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int SB = 128;   // scale block edge = K tile
@@ -124,18 +125,26 @@ __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // Core: 256 x 256 tiles, 4 waves (2 x 2) of 128 x 128, K tile 128.
+// Workgroup tile BM x BN x 128 with WGM x WGN waves; each wave owns a
+// WTM x WTN block of C inside one 128 x 128 scale block.  W4: 4 waves of
+// 128 x 128 (256 accumulators, one wave per SIMD).  W8: 8 waves of 128 x 64
+// (two waves per SIMD, which cover each other's LDS and barrier waits), with
+// the A fragments read one row-block ahead of their MFMAs and the stage copy
+// interleaved with the last row-block's MFMAs (ILV; see the split core).
+template <int BM_, int BN_, int WGM_, int WGN_, bool ILV_>
 struct F8Cfg {
-    static constexpr int BM = 256, BN = 256, BK = 128, NT = 256, WTM = 128, WTN = 128, FM = 8, FN = 8;
+    static constexpr int BM = BM_, BN = BN_, BK = 128, WGM = WGM_, WGN = WGN_, NT = 64 * WGM_ * WGN_;
+    static constexpr int WTM = BM / WGM, WTN = BN / WGN, FM = WTM / 16, FN = WTN / 16;
+    static constexpr bool ILV = ILV_;
     static constexpr int STAGE = (BM + BN) * BK;                  // bytes
     static constexpr int LDS_BYTES = 2 * STAGE;
     static constexpr int PER_T = STAGE / 16 / NT;                 // 16-byte chunks per thread
+    static_assert(SB % WTM == 0 || WTM % SB == 0, "a wave's rows lie in one scale block");
+    static_assert(WTM <= SB && WTN <= SB, "one scale per wave and K tile");
 };
-struct F8CfgSmall {
-    static constexpr int BM = 128, BN = 128, BK = 128, NT = 256, WTM = 64, WTN = 64, FM = 4, FN = 4;
-    static constexpr int STAGE = (BM + BN) * BK;
-    static constexpr int LDS_BYTES = 2 * STAGE;
-    static constexpr int PER_T = STAGE / 16 / NT;
-};
+using F8W4 = F8Cfg<256, 256, 2, 2, false>;
+using F8W4Small = F8Cfg<128, 128, 2, 2, false>;
+using F8W8 = F8Cfg<256, 256, 2, 4, true>;
 
 template <class CF>
 __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, const unsigned char *__restrict__ Aq,
@@ -149,26 +158,27 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = w >> 1, wn = w & 1;
+    const int wm = w / CF::WGN, wn = w % CF::WGN;
     const int kblocks = Kp / BK;
 
     // Stage kt: A rows m0.. (BM * 128 contiguous bytes), then B rows n0...
     constexpr int A_CH = BM * BK / 16 / CF::NT;
     const unsigned char *a_src = Aq + (long)m0 * BK + t * 16;
     const unsigned char *b_src = Bq + (long)n0 * BK + t * 16;
-    uint4 regs[PER_T];
+    // A native vector type (not HIP's uint4 struct), so that the array stays in registers.
+    u32x4 regs[PER_T];
     auto load_stage = [&](int kt) {
         const long ak = (long)kt * Mp * BK, bk = (long)kt * Np * BK;
 #pragma unroll
         for (int i = 0; i < PER_T; ++i) {
-            if (i < A_CH) regs[i] = *reinterpret_cast<const uint4 *>(a_src + ak + i * CF::NT * 16);
-            else regs[i] = *reinterpret_cast<const uint4 *>(b_src + bk + (i - A_CH) * CF::NT * 16);
+            if (i < A_CH) regs[i] = *reinterpret_cast<const u32x4 *>(a_src + ak + i * CF::NT * 16);
+            else regs[i] = *reinterpret_cast<const u32x4 *>(b_src + bk + (i - A_CH) * CF::NT * 16);
         }
     };
     auto store_stage = [&](int buf) {
         unsigned char *base = smem8 + buf * CF::STAGE + t * 16;
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<uint4 *>(base + i * CF::NT * 16) = regs[i];
+        for (int i = 0; i < PER_T; ++i) *reinterpret_cast<u32x4 *>(base + i * CF::NT * 16) = regs[i];
     };
 
     floatx4 acc[FM][FN];
@@ -179,39 +189,69 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
 
     load_stage(0);
     store_stage(0);
+    if constexpr (CF::ILV) load_stage(min(1, kblocks - 1));
     __syncthreads();
     const int frow = lane & 15, fq = lane >> 4;
     // Scale blocks of this wave: rows (m0 + wm*WTM) / 128, columns likewise.
     const int a_blk = (m0 + wm * CF::WTM) / SB, b_blk = (n0 + wn * CF::WTN) / SB;
+    // Fragment of plane row `row` (16 rows x 128 k, 32 bytes per lane).
+    auto frag = [&](const unsigned char *base, int row) {
+        const u32x4 lo = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq) ^ (row & 7)) << 4));
+        const u32x4 hi = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
+        return int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+    };
     for (int kt = 0; kt < kblocks; ++kt) {
         const int buf = kt & 1;
         const bool more = kt + 1 < kblocks;
-        if (more) load_stage(kt + 1);
+        if (!CF::ILV && more) load_stage(kt + 1);
         const int sa = ea[(long)a_blk * kblocks + kt] + 127, sb = eb[(long)b_blk * kblocks + kt] + 127;
         const unsigned char *as = smem8 + buf * CF::STAGE;
         const unsigned char *bs = as + BM * BK;
-        int8v fa[FM], fb[FN];
+        int8v fb[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i) {
-            const int row = wm * CF::WTM + i * 16 + frow;
-            const uint4 lo = *reinterpret_cast<const uint4 *>(as + row * BK + (((2 * fq) ^ (row & 7)) << 4));
-            const uint4 hi = *reinterpret_cast<const uint4 *>(as + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
-            fa[i] = int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+        for (int j = 0; j < FN; ++j) fb[j] = frag(bs, wn * CF::WTN + j * 16 + frow);
+        if constexpr (CF::ILV) {
+            // Row-block i's A fragment is read while row-block i-1 multiplies.
+            int8v fa_cur = frag(as, wm * CF::WTM + frow), fa_next;
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                if (i + 1 < FM) fa_next = frag(as, wm * CF::WTM + (i + 1) * 16 + frow);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa_cur, fb[j], acc[i][j], 0, 0, 0,
+                                                                                 sa, 0, sb);
+                if (i + 1 == FM / 2) {
+                    // The second half's scheduling region also carries the
+                    // stage copy: stage kt+1 from registers to the idle buffer,
+                    // then the loads of stage kt+2 (branch-free: the last step
+                    // stores into the idle buffer and re-loads the last stage).
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (i + 1 == FM) {
+                    store_stage(buf ^ 1);
+                    load_stage(min(kt + 2, kblocks - 1));
+                    constexpr int PER = (FM / 2) * FN / PER_T > 0 ? (FM / 2) * FN / PER_T : 1;
+#pragma unroll
+                    for (int c = 0; c < PER_T; ++c) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);   // MFMA
+                        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);     // DS write
+                        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
+                    }
+                }
+                fa_cur = fa_next;
+            }
+        } else {
+            int8v fa[FM];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fa[i] = frag(as, wm * CF::WTM + i * 16 + frow);
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sa,
+                                                                                 0, sb);
+            if (more) store_stage(buf ^ 1);
         }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int row = wn * CF::WTN + j * 16 + frow;
-            const uint4 lo = *reinterpret_cast<const uint4 *>(bs + row * BK + (((2 * fq) ^ (row & 7)) << 4));
-            const uint4 hi = *reinterpret_cast<const uint4 *>(bs + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
-            fb[j] = int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sa, 0,
-                                                                             sb);
-        if (more) store_stage(buf ^ 1);
         __syncthreads();
     }
 
@@ -300,21 +340,24 @@ int run(const Gemm<float> &g, hipStream_t s) {
     return rc ? rc : rc2;
 }
 
-const char *kNames[] = {"fp8_e4m3_blk128_256x256x128", "fp8_e4m3_blk128_128x128x128"};
+const char *kNames[] = {"fp8_e4m3_blk128_256x256x128_w2x2", "fp8_e4m3_blk128_128x128x128_w2x2",
+                        "fp8_e4m3_blk128_256x256x128_w2x4_ilv"};
+constexpr int kNumVariants = 3;
 
 }  // namespace
 
-int fp8_variant_count() { return 2; }
-const char *fp8_variant_name(int v) { return (v >= 0 && v < 2) ? kNames[v] : nullptr; }
+int fp8_variant_count() { return kNumVariants; }
+const char *fp8_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kNames[v] : nullptr; }
 
 int launch_fp8(const Gemm<float> &g, int variant, hipStream_t s) {
     if (variant < 0) {
         const long t256 = (long)((g.M + 255) / 256) * ((g.N + 255) / 256);
-        variant = t256 >= device_cu_count() ? 0 : 1;
+        variant = t256 >= device_cu_count() ? 2 : 1;
     }
     switch (variant) {
-        case 0: return run<F8Cfg>(g, s);
-        case 1: return run<F8CfgSmall>(g, s);
+        case 0: return run<F8W4>(g, s);
+        case 1: return run<F8W4Small>(g, s);
+        case 2: return run<F8W8>(g, s);
         default: return fail(UGEMM_ERR_ARG, "fp8: no variant %d", variant);
     }
 }
