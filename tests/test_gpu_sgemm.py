@@ -137,12 +137,13 @@ def test_uut_entry_and_packed_shim():
         assert case.judge(C)[0], (ta, tb)
 
 
-@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("variant", range(6))
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_fp32_every_variant(variant, ta, tb):
+    assert len(ug.variants(ug.MODE_FP32)) == 6
     ug.force_variant(ug.MODE_FP32, variant)
     try:
-        for (M, N, K) in ((130, 250, 70), (256, 256, 48)):
+        for (M, N, K) in ((130, 250, 70), (256, 256, 48), (520, 600, 33)):
             case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=31 + variant)
             C = run_host(case)
             ok, worst = case.judge(C)

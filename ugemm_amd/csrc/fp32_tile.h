@@ -32,9 +32,10 @@ __device__ __forceinline__ void pin_step(float (&acc)[TM][TN]) {
             asm volatile("" : "+v"(acc[i][j]), "+v"(acc[i][j + 1]), "+v"(acc[i][j + 2]), "+v"(acc[i][j + 3]));
 }
 
-template <int BM_, int BN_, int BK_, int TM_, int TN_>
+// MINB_: workgroups per CU the kernel is compiled for (register budget).
+template <int BM_, int BN_, int BK_, int TM_, int TN_, int MINB_ = 1>
 struct Fp32Cfg {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, TM = TM_, TN = TN_;
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, TM = TM_, TN = TN_, MINB = MINB_;
     static constexpr int TX = BN / TN, TY = BM / TM;
     static constexpr int NT = TX * TY;
     static_assert(TM % 4 == 0 && TN % 4 == 0, "micro-tile built from 4-wide groups");

@@ -301,7 +301,9 @@ int finish_plan(ugemm_grouped_plan *p, int count, int capture, ugemm_grouped_pla
             return hip_fail(e, "grouped: capture stream");
         }
     }
+    // A forced single-GEMM variant applies when the grouped kernel has it too.
     int v = forced_variant(UGEMM_MODE_FP32);
+    if (v >= kNumVariants) v = -1;
     if (v < 0) v = tuned_variant(TUNE_KEY_GROUPED, p->conv, false, count, (int)std::min(p->flops / 1e6, 2e9), 0);
     if (v < 0) v = default_variant(p);
     const int rc = build_schedule(p, v);
