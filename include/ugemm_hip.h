@@ -201,7 +201,10 @@ int ugemm_hip_last_error(void);
 const char *ugemm_hip_last_error_message(void);
 const char *ugemm_hip_status_string(int status);
 /* Time every compiled kernel variant on an (M, N, K, layout, mode) problem
- * and remember the fastest for that shape bucket. Allocates its own buffers. */
+ * -- and split-K counts when 128x128 tiles would leave CUs idle -- and
+ * remember the fastest for that shape bucket. Allocates its own buffers.
+ * The tuning replaces nothing in the reference (it has no tuner for this
+ * path); the cache file format is documented in tune.hip. */
 int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode);
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *plan);
 int ugemm_hip_tune_load(const char *path);
@@ -210,6 +213,12 @@ int ugemm_hip_tune_save(const char *path);
 int ugemm_hip_tune_lookup(char transa, char transb, int M, int N, int K, int mode);
 /* Force a variant for the next calls on this thread (-1 restores tuning/heuristics). */
 int ugemm_hip_force_variant(int mode, int variant);
+/* Force the split-K count (1..64; 1 = no split) of the next calls on this
+ * thread, mode -1 for DGEMM; 0 restores tuning/heuristics. A split call
+ * writes alpha = 1 partial products of K chunks to a workspace and adds
+ * them in a fixed order (bitwise reproducible). No counterpart in the
+ * reference; FP8 and convolutions do not split. */
+int ugemm_hip_force_splits(int mode, int splits);
 int ugemm_hip_variant_count(int mode);
 const char *ugemm_hip_variant_name(int mode, int variant);
 const char *ugemm_hip_version(void);

@@ -91,6 +91,7 @@ def load(path=None):
         "ugemm_hip_tune_save": (i, [cs]),
         "ugemm_hip_tune_lookup": (i, [c, c, i, i, i, i]),
         "ugemm_hip_force_variant": (i, [i, i]),
+        "ugemm_hip_force_splits": (i, [i, i]),
         "ugemm_hip_variant_count": (i, [i]),
         "ugemm_hip_variant_name": (cs, [i, i]),
         "ugemm_hip_version": (cs, []),
@@ -254,6 +255,11 @@ def variants(mode):
 
 def force_variant(mode, variant):
     return _raise_if(load().ugemm_hip_force_variant(mode, variant))
+
+
+def force_splits(mode, splits):
+    """Split-K count of this thread's next calls (mode -1: DGEMM); 0 = automatic."""
+    return _raise_if(load().ugemm_hip_force_splits(mode, splits))
 
 
 class GroupedPlan:

@@ -11,8 +11,9 @@
 // column l%16 (measured on gfx950: unlike the FP32-output forms, whose rows
 // are 4*(l/16) + i).
 //
-// Determinism: every element of C is a fixed sequence of MFMAs over k (no
-// split-K, no atomics), so repeated calls are bitwise identical.
+// Determinism: every element of C is a fixed sequence of MFMAs over k, and a
+// split-K call adds its partial slices in a fixed order (no atomics), so
+// repeated calls are bitwise identical.
 // Semantics as dgemm_cpu (ugemm.h:162-286) with the BLAS beta == 0 rule.
 #include "internal.h"
 #include "kernel_common.h"
@@ -29,11 +30,21 @@ struct DCfg {
     static constexpr int LDA = BM + 2, LDB = BN + 2;   // doubles; +2 keeps 16-byte alignment
 };
 
+// Split-K (gridDim.y > 1): block (tile, z) multiplies K columns
+// [z*kc, min(K, (z+1)*kc)) into the partial slice ws + z*M*N (row length N).
 template <class CF, bool TA, bool TB, int VEC>
-__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_mfma_kernel(int M, int N, int K, double alpha,
-                                                            const double *__restrict__ A, long lda,
-                                                            const double *__restrict__ B, long ldb, double beta,
-                                                            double *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_mfma_kernel(
+    int M, int N, int K_, double alpha_, const double *__restrict__ A_, long lda, const double *__restrict__ B_,
+    long ldb, double beta_, double *__restrict__ C_, long ldc_, int tiles_m, int tiles_n, int kc,
+    double *__restrict__ ws) {
+    const bool part = gridDim.y > 1;
+    const int kz0 = part ? (int)blockIdx.y * kc : 0;
+    const int K = part ? min(kc, K_ - kz0) : K_;
+    const double *__restrict__ A = TA ? A_ + (long)kz0 * lda : A_ + kz0;
+    const double *__restrict__ B = TB ? B_ + kz0 : B_ + (long)kz0 * ldb;
+    double *__restrict__ C = part ? ws + (long)blockIdx.y * M * N : C_;
+    const long ldc = part ? (long)N : ldc_;
+    const double alpha = part ? 1.0 : alpha_, beta = part ? 0.0 : beta_;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FM = CF::FM, FN = CF::FN;
     constexpr int LDA = CF::LDA, LDB = CF::LDB;
     __shared__ __attribute__((aligned(16))) double lds[2 * BK * (LDA + LDB)];
@@ -147,10 +158,17 @@ int launch_cfg(const Gemm<double> &g, hipStream_t s) {
     const long nblk = (long)tiles_m * tiles_n;
     if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "dgemm: too many tiles (%ld)", nblk);
     const bool vec = ugemm_aligned16(g.A) && ugemm_aligned16(g.B) && (g.lda % 2 == 0) && (g.ldb % 2 == 0);
-    dim3 grid((unsigned)nblk), block(CF::NT);
+    int kc = g.K;
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc);
+    double *ws = nullptr;
+    if (splits > 1) {
+        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(double), s);
+        if (rc) return rc;
+    }
+    dim3 grid((unsigned)nblk, (unsigned)splits), block(CF::NT);
 #define UGEMM_L(TA, TB, VEC)                                                                                \
     hipLaunchKernelGGL((dgemm_mfma_kernel<CF, TA, TB, VEC>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, \
-                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n)
+                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n, kc, ws)
     timing_begin(s);
     if (vec) {
         if (!g.ta && !g.tb) UGEMM_L(false, false, 2);
@@ -165,8 +183,14 @@ int launch_cfg(const Gemm<double> &g, hipStream_t s) {
     }
     timing_end(s);
 #undef UGEMM_L
-    UGEMM_HIP_TRY(hipGetLastError());
-    return UGEMM_OK;
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "dgemm launch");
+    if (splits > 1) {
+        if (!rc) rc = launch_splitk_reduce_f64(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, s);
+        const int rc2 = stream_free(ws, s);
+        if (!rc) rc = rc2;
+    }
+    return rc;
 }
 
 }  // namespace

@@ -395,7 +395,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                                                        const unsigned *__restrict__ amax_a,
                                                        const unsigned *__restrict__ amax_b, Epilogue epi,
                                                        const float *__restrict__ Af, long lda,
-                                                       const float *__restrict__ Bf, long ldb, int K, int vec_ab) {
+                                                       const float *__restrict__ Bf, long ldb, int K, int vec_ab,
+                                                       int kt_per, float *__restrict__ ws) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
@@ -550,16 +551,24 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    const int ktiles = Kp / BK;
+    // K tiles [kt0, ktiles) of this block: all of them, or (split-K,
+    // gridDim.y > 1) the kt_per tiles of split blockIdx.y, whose alpha = 1,
+    // beta = 0 product goes to the partial slice ws + z*M*N (row length N).
+    const bool part = gridDim.y > 1;
+    const int kt0 = part ? (int)blockIdx.y * kt_per : 0;
+    const int ktiles = part ? min(Kp / BK, kt0 + kt_per) : Kp / BK;
+    float *const Co = part ? ws + (long)blockIdx.y * M * N : C;
+    const long ldo = part ? (long)N : ldc;
+    const float al = part ? 1.0f : alpha, be = part ? 0.0f : beta;
     constexpr bool ILV = CF::ILV && !FA && !FB;
-    load_stage(0);
+    load_stage(kt0);
     store_stage(0);
-    if constexpr (ILV) load_stage(min(1, ktiles - 1));
+    if constexpr (ILV) load_stage(min(kt0 + 1, ktiles - 1));
     __syncthreads();
 
     const int frow = lane & 15, fc = lane >> 4;
-    for (int kt = 0; kt < ktiles; ++kt) {
-        const int buf = (STAGES == 2) ? (kt & 1) : 0;
+    for (int kt = kt0; kt < ktiles; ++kt) {
+        const int buf = (STAGES == 2) ? ((kt - kt0) & 1) : 0;
         const bool more = kt + 1 < ktiles;
         if (!ILV && more) load_stage(kt + 1);
         const E *as = lds + buf * STAGE;
@@ -652,7 +661,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     // 256 accumulators per lane there is no room for the batch (it spills):
     // those variants interleave.
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
-    const bool has_epi = epi.active();
+    const bool has_epi = !part && epi.active();
     constexpr bool BATCH = FM * FN * 4 < 256;
     constexpr int JB = !BATCH ? 1 : (CF::NT >= 1024 ? 1 : FN);
 #pragma unroll
@@ -661,13 +670,13 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     for (int j0 = 0; j0 < FN; j0 += JB) {
         const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
         float cv[JB][4];
-        if (BATCH && beta != 0.0f) {
+        if (BATCH && be != 0.0f) {
 #pragma unroll
             for (int jj = 0; jj < JB; ++jj) {
                 const int n = n0 + wn * CF::WTN + (j0 + jj) * 16 + (lane & 15);
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    cv[jj][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
+                    cv[jj][r] = (interior || (mb + r < M && n < N)) ? Co[(long)(mb + r) * ldo + n] : 0.0f;
             }
         }
 #pragma unroll
@@ -682,8 +691,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
                 if (interior || (m < M && n < N)) {
                     float a = acc[i][j][r];
                     if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
-                    float *cp = C + (long)m * ldc + n;
-                    float v = (beta == 0.0f) ? alpha * a : fmaf(alpha, a, beta * (BATCH ? cv[jj][r] : *cp));
+                    float *cp = Co + (long)m * ldo + n;
+                    float v = (be == 0.0f) ? al * a : fmaf(al, a, be * (BATCH ? cv[jj][r] : *cp));
                     if (has_epi) v = apply_epilogue(epi, v, m, n);
                     *cp = v;
                 }
@@ -824,13 +833,27 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
                                   : (ugemm_aligned16(X) && ld % 4 == 0);
     };
     const int vec_ab = (vec_ok(g.A, g.lda, FA) ? 1 : 0) | (vec_ok(g.B, g.ldb, FB) ? 2 : 0);
+    // Split-K: chunks of whole 32-wide K tiles.
+    int kc = Kp;
+    const int splits = split_count(g.splits, (long)tiles_m * tiles_n, Kp, CF::BK, &kc);
+    float *ws = nullptr;
+    if (splits > 1) {
+        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
+        if (rc) return rc;
+    }
     timing_begin(s);
-    hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n)), dim3(CF::NT), CF::LDS_BYTES, s, g.M, g.N, Kp, Ap,
-                       a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n, amax_a, amax_b,
-                       g.epi, g.A, g.lda, g.B, g.ldb, g.K, vec_ab);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(CF::NT), CF::LDS_BYTES, s,
+                       g.M, g.N, Kp, Ap, a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n,
+                       amax_a, amax_b, g.epi, g.A, g.lda, g.B, g.ldb, g.K, vec_ab, kc / CF::BK, ws);
     timing_end(s);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? UGEMM_OK : hip_fail(e, "split core launch");
+    int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "split core launch");
+    if (splits > 1) {
+        if (!rc) rc = launch_splitk_reduce_f32(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, g.epi, s);
+        const int rc2 = stream_free(ws, s);
+        if (!rc) rc = rc2;
+    }
+    return rc;
 }
 
 // Workspace: NP planes of A (Mp x Kp) and of B (Np x Kp) -- none for an

@@ -48,6 +48,7 @@ struct Gemm {
     T *C; long ldc;
     bool ta, tb;
     Epilogue epi{};
+    int splits = 0;   // split-K count: 0 = the launcher picks (split_count), 1 = none
 };
 
 // ---- status (runtime.hip) ----
@@ -102,12 +103,28 @@ int launch_sgemv(char trans, int M, int N, float alpha, const float *A, int lda,
 int launch_dgemv(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
                  double beta, double *y, int incy, hipStream_t s);
 
+// ---- split-K (splitk.hip) ----
+// A GEMM whose tile grid leaves CUs idle is split along K: block (tile, z)
+// computes alpha = 1, beta = 0 partial products over K columns
+// [z*kc, min(K, (z+1)*kc)) into workspace slice z (M x N, row length N), and
+// a reduction adds the slices in z order -- a fixed order, so results are
+// bitwise reproducible -- then applies alpha, beta and the epilogue.
+// split_count: the number of splits for a grid of `tiles` tiles (requested
+// > 0: that many, capped so that every split has a K chunk); *kc is set to a
+// multiple of `kalign`.
+int split_count(int requested, long tiles, int K, int kalign, int *kc);
+int launch_splitk_reduce_f32(int M, int N, int splits, float alpha, const float *ws, float beta, float *C, long ldc,
+                             const Epilogue &epi, hipStream_t s);
+int launch_splitk_reduce_f64(int M, int N, int splits, double alpha, const double *ws, double beta, double *C,
+                             long ldc, hipStream_t s);
+
 // ---- tuning (tune.hip) ----
 // Tuning-table key of grouped plans (variants = the 4 FP32 tile shapes).
 constexpr int TUNE_KEY_GROUPED = 16;
-int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K);   // -1 if none
+int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits = nullptr);   // -1 if none
 int forced_variant(int mode);                                         // -1 if none
-void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms);
+int forced_splits(int mode);                                          // 0 if none
+void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms, int splits = 0);
 
 // ---- launch timing (runtime.hip) ----
 // When enabled (ugemm_hip_timing_enable), the dominant kernel of every call

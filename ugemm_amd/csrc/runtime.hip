@@ -233,10 +233,13 @@ static int normalize(char major, char transa, char transb, int M, int N, int K, 
 
 // ---------------------------------------------------------------------------
 // Device-pointer execution of a normalised GEMM.
-static int choose_variant(const Gemm<float> &g, int mode) {
-    int v = forced_variant(mode);
-    if (v >= 0) return v;
-    v = tuned_variant(mode, g.ta, g.tb, g.M, g.N, g.K);
+// Variant and split-K count: forced (ugemm_hip_force_variant/_splits), else
+// the tuning table's entry for the shape bucket, else the built-in choice.
+static int choose_variant(const Gemm<float> &g, int mode, int *splits) {
+    int v = forced_variant(mode), tsplits = 0;
+    const int fs = forced_splits(mode);
+    if (v < 0) v = tuned_variant(mode, g.ta, g.tb, g.M, g.N, g.K, &tsplits);
+    *splits = fs > 0 ? fs : tsplits;
     if (v >= 0) return v;
     switch (mode) {
         case UGEMM_MODE_FP32: return fp32_pick_variant(g);
@@ -253,14 +256,19 @@ int run_sgemm(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
     }
     if (g.epi.active() && (mode == UGEMM_MODE_FP8_BLOCK))
         return fail(UGEMM_ERR_UNSUPPORTED, "bias/activation epilogue: not provided for the FP8 form");
-    if (variant < 0) variant = choose_variant(g, mode);
+    Gemm<float> gs = g;
+    if (variant < 0) {
+        variant = choose_variant(g, mode, &gs.splits);
+    } else if (gs.splits == 0 && forced_splits(mode) > 0) {
+        gs.splits = forced_splits(mode);
+    }
     switch (mode) {
-        case UGEMM_MODE_FP32: return launch_fp32(g, variant, s);
+        case UGEMM_MODE_FP32: return launch_fp32(gs, variant, s);
         case UGEMM_MODE_BF16X1:
         case UGEMM_MODE_BF16X3:
         case UGEMM_MODE_BF16X6:
-        case UGEMM_MODE_FP16X3: return launch_mfma(g, mode, variant, s);
-        case UGEMM_MODE_FP8_BLOCK: return launch_fp8(g, variant, s);
+        case UGEMM_MODE_FP16X3: return launch_mfma(gs, mode, variant, s);
+        case UGEMM_MODE_FP8_BLOCK: return launch_fp8(gs, variant, s);
         default: return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
     }
 }
@@ -275,7 +283,9 @@ int run_dgemm(const Gemm<double> &g, int variant, hipStream_t s) {
         variant = forced_variant(-1);
         if (variant < 0) variant = dgemm_pick_variant(g);
     }
-    return launch_dgemm(g, variant, s);
+    Gemm<double> gs = g;
+    if (gs.splits == 0) gs.splits = forced_splits(-1);
+    return launch_dgemm(gs, variant, s);
 }
 
 // ---------------------------------------------------------------------------

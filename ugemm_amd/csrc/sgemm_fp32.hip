@@ -7,16 +7,27 @@
 namespace ugemm {
 namespace {
 
+// Split-K (gridDim.y > 1): block (tile, z) multiplies K columns
+// [z*kc, min(K, (z+1)*kc)) into the partial slice ws + z*M*N (row length N).
 template <class CF, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(CF::NT, CF::MINB) void sgemm_fp32_kernel(int M, int N, int K, float alpha,
                                                             const float *__restrict__ A, long lda,
                                                             const float *__restrict__ B, long ldb, float beta,
                                                             float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
-                                                            Epilogue epi) {
+                                                            Epilogue epi, int kc, float *__restrict__ ws) {
     __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    fp32_tile<CF, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm * CF::BM, tn * CF::BN, lds, true, epi);
+    if (gridDim.y > 1) {
+        const int k0 = blockIdx.y * kc, kz = min(kc, K - k0);
+        const float *Az = TA ? A + (long)k0 * lda : A + k0;
+        const float *Bz = TB ? B + k0 : B + (long)k0 * ldb;
+        fp32_tile<CF, TA, TB, VEC>(M, N, kz, 1.0f, Az, lda, Bz, ldb, 0.0f, ws + (long)blockIdx.y * M * N, N,
+                                   tm * CF::BM, tn * CF::BN, lds, true, Epilogue());
+    } else {
+        fp32_tile<CF, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm * CF::BM, tn * CF::BN, lds, true,
+                                   epi);
+    }
 }
 
 // Implicit-GEMM convolution: C (M = output channels x N = output pixels) =
@@ -53,12 +64,20 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     const int tiles_n = (g.N + CF::BN - 1) / CF::BN;
     const long nblk = (long)tiles_m * tiles_n;
     if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "sgemm: too many tiles (%ld)", nblk);
-    // 128-bit loads need 16-byte aligned bases and leading dimensions that are multiples of 4.
+    // 128-bit loads need 16-byte aligned bases and leading dimensions that are multiples of 4
+    // (split-K chunk offsets are multiples of 64 columns, which keeps that).
     const bool vec = ugemm_aligned16(g.A) && ugemm_aligned16(g.B) && (g.lda % 4 == 0) && (g.ldb % 4 == 0);
-    dim3 grid((unsigned)nblk), block(CF::NT);
+    int kc = g.K;
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc);
+    float *ws = nullptr;
+    if (splits > 1) {
+        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
+        if (rc) return rc;
+    }
+    dim3 grid((unsigned)nblk, (unsigned)splits), block(CF::NT);
 #define UGEMM_L(TA, TB, VEC)                                                                             \
     hipLaunchKernelGGL((sgemm_fp32_kernel<CF, TA, TB, VEC>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, \
-                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi)
+                       g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, kc, ws)
     timing_begin(s);
     if (vec) {
         if (!g.ta && !g.tb) UGEMM_L(false, false, 4);
@@ -73,8 +92,14 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     }
     timing_end(s);
 #undef UGEMM_L
-    UGEMM_HIP_TRY(hipGetLastError());
-    return UGEMM_OK;
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "sgemm launch");
+    if (splits > 1) {
+        if (!rc) rc = launch_splitk_reduce_f32(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, g.epi, s);
+        const int rc2 = stream_free(ws, s);
+        if (!rc) rc = rc2;
+    }
+    return rc;
 }
 
 template <class CF>

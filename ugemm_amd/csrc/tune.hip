@@ -1,13 +1,15 @@
-// Autotuner: times every compiled kernel variant of a mode on a problem and
+// Autotuner: times every compiled kernel variant of a mode on a problem --
+// and, when the tile grid is too small to fill the GPU, split-K counts -- and
 // remembers the fastest per shape bucket; the table persists as a small text
 // file (safe to delete: lookups then fall back to the built-in heuristics).
 //
 // Bucket key: (mode, transa, transb, ceil(log2 M), ceil(log2 N), ceil(log2 K));
 // grouped plans use key TUNE_KEY_GROUPED with (count, MFLOP, 0) in place of
 // (M, N, K).
-// File format (version 1):
-//   ugemm-hip-tune 1 <gcnArch> <CUs>
-//   <mode> <ta> <tb> <lgM> <lgN> <lgK> <variant> <ms>
+// File format (version 2; version-1 files, which have no <splits> column,
+// are read with splits = 0, the built-in choice):
+//   ugemm-hip-tune 2 <gcnArch> <CUs>
+//   <mode> <ta> <tb> <lgM> <lgN> <lgK> <variant> <ms> <splits>
 // Entries from a different architecture or CU count are ignored on load.
 #include "internal.h"
 
@@ -26,6 +28,7 @@ namespace {
 struct Entry {
     int variant;
     float ms;
+    int splits;   // 0: the launcher's choice
 };
 using Key = std::tuple<int, int, int, int, int, int>;
 std::mutex g_mu;
@@ -33,6 +36,7 @@ std::map<Key, Entry> g_table;
 // Forced variant per mode (slot UGEMM_MODE_COUNT: DGEMM); stored + 1 so that
 // zero-initialisation means "none".
 thread_local int t_forced_p1[UGEMM_MODE_COUNT + 1] = {};
+thread_local int t_forced_splits[UGEMM_MODE_COUNT + 1] = {};   // 0: none
 
 int lg(int x) {
     int b = 0;
@@ -67,17 +71,19 @@ void arch_info(char *arch, size_t n, int *cus) {
 
 }  // namespace
 
-int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K) {
+int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_table.find(key(mode, ta, tb, M, N, K));
+    if (splits) *splits = it == g_table.end() ? 0 : it->second.splits;
     return it == g_table.end() ? -1 : it->second.variant;
 }
 
 int forced_variant(int mode) { return t_forced_p1[slot(mode)] - 1; }
+int forced_splits(int mode) { return t_forced_splits[slot(mode)]; }
 
-void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms) {
+void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms, int splits) {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_table[key(mode, ta, tb, M, N, K)] = Entry{variant, ms};
+    g_table[key(mode, ta, tb, M, N, K)] = Entry{variant, ms, splits};
 }
 
 }  // namespace ugemm
@@ -90,6 +96,13 @@ int ugemm_hip_force_variant(int mode, int variant) {
     if (mode < -1 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
     if (variant >= ugemm_hip_variant_count(mode)) return fail(UGEMM_ERR_ARG, "mode %d has no variant %d", mode, variant);
     t_forced_p1[slot(mode)] = variant < 0 ? 0 : variant + 1;
+    return UGEMM_OK;
+}
+
+int ugemm_hip_force_splits(int mode, int splits) {
+    if (mode < -1 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    if (splits < 0 || splits > 64) return fail(UGEMM_ERR_ARG, "splits must be in [0, 64] (got %d)", splits);
+    t_forced_splits[slot(mode)] = splits;
     return UGEMM_OK;
 }
 
@@ -111,9 +124,14 @@ int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode
     float *A = nullptr, *B = nullptr, *C = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int best = -1;
+    int best = -1, best_splits = 0;
     float best_ms = 0;
     rc = UGEMM_OK;
+    // Split counts worth timing: only when 128x128 tiles leave CUs idle
+    // (0 = the built-in choice, always timed; FP8 does not split).
+    const long tiles128 = (long)((M + 127) / 128) * ((N + 127) / 128);
+    const int split_set[] = {0, 1, 2, 4, 8, 16};
+    const int nsplit = (tiles128 < device_cu_count() && mode != UGEMM_MODE_FP8_BLOCK) ? 6 : 1;
     do {
         hipError_t e;
         if ((e = hipMalloc(&A, na * 4)) || (e = hipMalloc(&B, nb * 4)) || (e = hipMalloc(&C, nc * 4)) ||
@@ -128,21 +146,25 @@ int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode
         Gemm<float> g{M, N, K, 1.0f, 0.0f, A, lda, B, ldb, C, ldc, ta, tb};
         const double flop = 2.0 * M * N * K;
         const int reps = flop > 1e11 ? 2 : (flop > 1e9 ? 5 : 20);
-        for (int v = 0; v < nv; ++v) {
-            if (run_sgemm(g, mode, v, s)) continue;   // variant not applicable here
-            hipEventRecord(e0, s);
-            for (int r = 0; r < reps; ++r) run_sgemm(g, mode, v, s);
-            hipEventRecord(e1, s);
-            if ((e = hipEventSynchronize(e1))) {
-                rc = hip_fail(e, "tune: timing");
-                break;
-            }
-            float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
-            ms /= reps;
-            if (best < 0 || ms < best_ms) {
-                best = v;
-                best_ms = ms;
+        for (int v = 0; v < nv && !rc; ++v) {
+            for (int si = 0; si < nsplit; ++si) {
+                g.splits = split_set[si];
+                if (run_sgemm(g, mode, v, s)) continue;   // variant not applicable here
+                hipEventRecord(e0, s);
+                for (int r = 0; r < reps; ++r) run_sgemm(g, mode, v, s);
+                hipEventRecord(e1, s);
+                if ((e = hipEventSynchronize(e1))) {
+                    rc = hip_fail(e, "tune: timing");
+                    break;
+                }
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                if (best < 0 || ms < best_ms) {
+                    best = v;
+                    best_splits = g.splits;
+                    best_ms = ms;
+                }
             }
         }
     } while (0);
@@ -157,7 +179,7 @@ int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode
     hipFree(C);
     if (rc) return rc;
     if (best < 0) return fail(UGEMM_ERR_UNSUPPORTED, "tune: no variant ran");
-    tune_record(mode, ta, tb, M, N, K, best, best_ms);
+    tune_record(mode, ta, tb, M, N, K, best, best_ms, best_splits);
     return best;
 }
 
@@ -170,13 +192,13 @@ int ugemm_hip_tune_save(const char *path) {
     char arch[64];
     int cus;
     arch_info(arch, sizeof arch, &cus);
-    fprintf(f, "ugemm-hip-tune 1 %s %d\n", arch, cus);
+    fprintf(f, "ugemm-hip-tune 2 %s %d\n", arch, cus);
     {
         std::lock_guard<std::mutex> lk(g_mu);
         for (auto &kv : g_table) {
             const Key &k = kv.first;
-            fprintf(f, "%d %d %d %d %d %d %d %.6f\n", std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k),
-                    std::get<4>(k), std::get<5>(k), kv.second.variant, kv.second.ms);
+            fprintf(f, "%d %d %d %d %d %d %d %.6f %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                    std::get<3>(k), std::get<4>(k), std::get<5>(k), kv.second.variant, kv.second.ms, kv.second.splits);
         }
     }
     if (fclose(f)) return fail(UGEMM_ERR_TUNE_FILE, "cannot write %s", path);
@@ -192,20 +214,21 @@ int ugemm_hip_tune_load(const char *path) {
     char arch[64], farch[64];
     int cus, fcus, ver;
     arch_info(arch, sizeof arch, &cus);
-    if (fscanf(f, "ugemm-hip-tune %d %63s %d", &ver, farch, &fcus) != 3 || ver != 1) {
+    if (fscanf(f, "ugemm-hip-tune %d %63s %d", &ver, farch, &fcus) != 3 || (ver != 1 && ver != 2)) {
         fclose(f);
-        return fail(UGEMM_ERR_TUNE_FILE, "%s: not a version-1 tuning cache", path);
+        return fail(UGEMM_ERR_TUNE_FILE, "%s: not a version-1 or version-2 tuning cache", path);
     }
     int loaded = 0;
     if (!strcmp(arch, farch) && cus == fcus) {
-        int m, a, b, lm, ln, lk, v;
+        int m, a, b, lm, ln, lk, v, sp = 0;
         float ms;
         while (fscanf(f, "%d %d %d %d %d %d %d %f", &m, &a, &b, &lm, &ln, &lk, &v, &ms) == 8) {
+            if (ver == 2 && fscanf(f, "%d", &sp) != 1) break;
             const int nv = (m == TUNE_KEY_GROUPED) ? ugemm_hip_variant_count(UGEMM_MODE_FP32)
                                                    : ((m < -1 || m >= UGEMM_MODE_COUNT) ? 0 : ugemm_hip_variant_count(m));
-            if (v < 0 || v >= nv) continue;
+            if (v < 0 || v >= nv || sp < 0 || sp > 64) continue;
             std::lock_guard<std::mutex> lk2(g_mu);
-            g_table[Key(m, a, b, lm, ln, lk)] = Entry{v, ms};
+            g_table[Key(m, a, b, lm, ln, lk)] = Entry{v, ms, sp};
             ++loaded;
         }
     }
