@@ -48,6 +48,27 @@ def test_split_k_parity(mode, splits, ta, tb):
         ug.force_splits(mode, 0)
 
 
+@pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
+@pytest.mark.parametrize("variant", [4, 6])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_split_k_with_eight_wave_variants(mode, variant, ta, tb):
+    """Split K chunks that start at a non-zero K tile, through the 8-wave
+    variants: the interleaved stage copy (variant 6, 3 x 3 tiles) and an
+    operand split in the core from either FP32 layout (300 or 1100 rows: one
+    side of at most two 256-wide tiles)."""
+    ug.force_variant(mode, variant)
+    ug.force_splits(mode, 3)
+    try:
+        for (M, N, K) in ((600, 700, 3000), (300, 1100, 2500), (1100, 257, 2100)):
+            case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=M + N + K + variant)
+            C = run_host(case, mode)
+            ok, worst = case.judge(C, mode=mode)
+            assert ok, f"variant {variant} {ta}{tb} {M}x{N}x{K}: worst/bound {worst:.3g}"
+    finally:
+        ug.force_splits(mode, 0)
+        ug.force_variant(mode, -1)
+
+
 @pytest.mark.parametrize("mode", SPLIT_MODES)
 def test_split_k_bitwise_reproducible_and_beta_zero(mode):
     M, N, K = 150, 140, 4100
