@@ -119,7 +119,9 @@ int ugemm_hip_sgemm_ex_dev(char major, char transa, char transb, int M, int N, i
  * (sgemm_gl1.h:191-218) and ocl_convolution_LReLU (sgemm_ocl1.h:302-337):
  * host pointers, synchronous, slope 0.1; status in ugemm_hip_last_error().
  * ugemm_hip_conv2d_dev: device pointers, stream-ordered, act/slope/bias
- * selectable (d_bias may be NULL); mode UGEMM_MODE_FP32. */
+ * selectable (d_bias may be NULL); mode UGEMM_MODE_FP32 (im2col gathered in
+ * the FP32 tile loader) or a split form (BF16X1/X3/X6, FP16X3: the im2col
+ * operand is gathered into its planes); FP8 is refused (UNSUPPORTED). */
 void convolution_LReLU_hip(float *inputs, int ich, int w, int h, float *weights, int k, int pad, int stride,
                            float *outputs, int ch, float *bias);
 int ugemm_hip_conv2d_dev(const float *d_in, int ich, int w, int h, const float *d_weights, int k, int pad,

@@ -7,6 +7,7 @@ out = out > 0 ? out : 0.1 * out.  Acceptance: the FP32 elementwise bound
 shrinks errors).  The GEMM epilogue is checked against the oracle's exact
 product followed by the same bias/activation."""
 import ctypes
+import zlib
 
 import numpy as np
 import pytest
@@ -97,7 +98,30 @@ def test_conv2d_argument_errors():
     assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 0, d.ptr, 1, None, 1, 0.1, 0, None) == ug.ERR_ARG
     assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 7, 0.1, 0, None) == ug.ERR_ARG
     assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 1, 0.1,
-                                  ug.MODE_BF16X3, None) == ug.ERR_UNSUPPORTED
+                                  ug.MODE_FP8_BLOCK, None) == ug.ERR_UNSUPPORTED
+    assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 1, 0.1, 42, None) == ug.ERR_ARG
+
+
+@pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1, ug.MODE_BF16X6])
+@pytest.mark.parametrize("shape", [(3, 17, 13, 3, 1, 1, 40), (16, 28, 28, 3, 1, 1, 64), (5, 20, 31, 5, 2, 2, 9),
+                                   (1, 9, 9, 1, 0, 1, 300), (32, 14, 14, 3, 1, 1, 300)])
+def test_conv2d_split_forms(mode, shape):
+    """Convolutions in the split forms: the im2col operand gathered into
+    its planes, then the split core with bias and LeakyReLU; bound of the
+    form (helpers.MODE_EXTRA) around the FP64 convolution."""
+    ich, h, w, k, pad, stride, ch = shape
+    g = orc.rng(zlib.crc32(f"{mode}{shape}".encode()))
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,))
+    hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
+    dx, dw, db = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(wts), hip.DeviceArray.from_host(bias)
+    dout = hip.DeviceArray.from_host(np.full((ch, hcol * wcol), np.nan, np.float32))
+    ug.conv2d_dev(dx, ich, w, h, dw, k, pad, stride, dout, ch, db, act=ug.ACT_LEAKY_RELU, slope=0.1, mode=mode)
+    hip.synchronize()
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=1, slope=0.1)
+    ok, worst = orc.check_bound(dout.download(), exact, scale, orc.tolerance(ich * k * k, extra=MODE_EXTRA[mode]))
+    assert ok, f"mode {mode} {shape}: worst/bound {worst:.3g}"
 
 
 # ---------------------------------------------------------------------------

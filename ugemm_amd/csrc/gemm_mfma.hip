@@ -280,6 +280,55 @@ __global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ 
     if (col < rows && v > 0.0f) atomicMax(&amax[col], __float_as_uint(v));
 }
 
+// Implicit im2col operand of a convolution (ConvDesc): element (n, k) of the
+// N x K operand (opB transposed) is in[c][ho*stride - pad + i][wo*stride - pad + j]
+// for n = ho*wcol + wo and k = (c, i, j) = c*kk*kk + i*kk + j, zero outside the
+// image and beyond N x K; ktab[k] = {c*H*W + i*W + j, (i << 16) | j}.  The
+// gather follows sgemm_gl1.h:166-190 (see oracle_im2col).
+__device__ __forceinline__ float im2col_at(const ConvDesc &cd, int n, int k, int N, int K) {
+    if (n >= N || k >= K) return 0.0f;
+    const int ho = n / cd.wcol, wo = n - ho * cd.wcol;
+    const int h0 = ho * cd.stride - cd.pad, w0 = wo * cd.stride - cd.pad;
+    const int2 e = cd.ktab[k];
+    const int hi = h0 + (e.y >> 16), wi = w0 + (e.y & 0xffff);
+    return ((unsigned)hi < (unsigned)cd.H && (unsigned)wi < (unsigned)cd.W) ? cd.in[(long)h0 * cd.W + w0 + e.x] : 0.0f;
+}
+
+// Planes of the im2col operand, gathered straight from the image (no im2col
+// matrix in HBM): one thread per 8 k of one pixel n, as split_prep_kcont.
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_im2col(ConvDesc cd, int N, int K, typename FMT::E *__restrict__ out,
+                                                         long plane_stride, int rows_pad, int k_pad,
+                                                         const unsigned *__restrict__ amax) {
+    const long chunks_per_row = k_pad / 8;
+    const long total = (long)rows_pad * chunks_per_row;
+    for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(f / chunks_per_row);
+        const int k = (int)(f - (long)r * chunks_per_row) * 8;
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = im2col_at(cd, r, k + j, N, K);
+        if constexpr (FMT::SCALED) {
+            const int e = -row_exp(amax[r]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
+        }
+        write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
+    }
+}
+
+// Largest magnitude of every pixel column of the im2col operand (scaled
+// formats): thread n of block (bx, by) scans k in [by*kspan, +kspan).
+__global__ __launch_bounds__(256) void row_amax_im2col(ConvDesc cd, int N, int K, int kspan,
+                                                       unsigned *__restrict__ amax) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    const int k0 = blockIdx.y * kspan, k1 = min(K, k0 + kspan);
+    float m = 0.0f;
+    for (int k = k0; k < k1; ++k) m = fmaxf(m, fabsf(im2col_at(cd, n, k, N, K)));
+    if (m > 0.0f) atomicMax(&amax[n], __float_as_uint(m));
+}
+
 // ---------------------------------------------------------------------------
 // MFMA core over prepared planes.
 
@@ -935,6 +984,66 @@ int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
     }
 }
 
+// Convolution as a split-form GEMM: C (channels x pixels) = weights (M x K,
+// k-contiguous) times the im2col operand, whose planes are gathered from the
+// image by split_prep_im2col; then the core with the epilogue.
+template <class CF>
+int run_cfg_conv(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
+    using FMT = typename CF::FMT;
+    using E = typename FMT::E;
+    constexpr int NP = CF::NP;
+    const long Mp = round_up(g.M, CF::BM), Np = round_up(g.N, CF::BN), Kp = round_up(g.K, 64);
+    const long a_plane = Mp * Kp, b_plane = Np * Kp;
+    const size_t a_bytes = (size_t)NP * a_plane * sizeof(E), b_bytes = (size_t)NP * b_plane * sizeof(E);
+    const size_t amax_bytes = FMT::SCALED ? (size_t)(Mp + Np) * sizeof(unsigned) : 0;
+    const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
+    if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
+    if (FMT::SCALED && (long)g.M * ((g.K + 2047) / 2048) > 0x7fffffffL)
+        return fail(UGEMM_ERR_ARG, "too many row blocks");
+    void *ws = nullptr;
+    int rc = stream_alloc(&ws, a_bytes + b_bytes + amax_bytes + 256, s);
+    if (rc) return rc;
+    E *Ap = (E *)ws, *Bp = (E *)((char *)ws + a_bytes);
+    unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + a_bytes + b_bytes) : nullptr;
+    unsigned *amax_b = FMT::SCALED ? amax_a + Mp : nullptr;
+    if (FMT::SCALED)
+        hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
+                           s, amax_a, Mp + Np);
+    rc = prep_operand<NP, FMT>(g.A, g.lda, true, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s);
+    if (!rc) {
+        constexpr int KSPAN = 512;
+        if (FMT::SCALED)
+            hipLaunchKernelGGL(row_amax_im2col, dim3((unsigned)((g.N + 255) / 256), (unsigned)((g.K + KSPAN - 1) / KSPAN)),
+                               dim3(256), 0, s, cd, g.N, g.K, KSPAN, amax_b);
+        const long total = Np * (Kp / 8);
+        const long blocks = std::max<long>(1, std::min<long>((total + 255) / 256, 64L * device_cu_count()));
+        hipLaunchKernelGGL((split_prep_im2col<NP, FMT>), dim3((unsigned)blocks), dim3(256), 0, s, cd, g.N, g.K, Bp,
+                           b_plane, (int)Np, (int)Kp, (const unsigned *)amax_b);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = hip_fail(e, "conv: operand gather");
+    }
+    if (!rc)
+        rc = launch_core<CF, FUSE_NONE, FUSE_NONE>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m,
+                                                   tiles_n, amax_a, amax_b, s);
+    const int rc2 = stream_free(ws, s);
+    return rc ? rc : rc2;
+}
+
+template <int NP, class FMT = FmtBf16>
+int launch_np_conv(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s) {
+    using T = Table<NP, FMT>;
+    switch (variant) {
+        case 0: return run_cfg_conv<typename T::V0>(g, cd, s);
+        case 1: return run_cfg_conv<typename T::V1>(g, cd, s);
+        case 2: return run_cfg_conv<typename T::V2>(g, cd, s);
+        case 3: return run_cfg_conv<typename T::V3>(g, cd, s);
+        case 4: return run_cfg_conv<typename T::V4>(g, cd, s);
+        case 5: return run_cfg_conv<typename T::V5>(g, cd, s);
+        case 6: return run_cfg_conv<typename T::V6>(g, cd, s);
+        default: return fail(UGEMM_ERR_ARG, "split form: no variant %d", variant);
+    }
+}
+
 template <int NP, class FMT = FmtBf16>
 const char *name_np(int v) {
     using T = Table<NP, FMT>;
@@ -984,6 +1093,21 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= cus / 2) return 1;
     return 3;
+}
+
+int launch_mfma_conv(const Gemm<float> &g, const ConvDesc &cd, int mode, int variant, hipStream_t s) {
+    // No operand is split in the core here (the im2col operand is gathered
+    // into planes), so the picker's fusing variant gives way to the
+    // plane-streaming one.
+    int v = variant >= 0 ? variant : mfma_pick_variant(g, mode);
+    if (variant < 0 && v == 4) v = planes_of(mode) == 2 ? 6 : 5;
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: return launch_np_conv<1>(g, cd, v, s);
+        case UGEMM_MODE_BF16X3: return launch_np_conv<2>(g, cd, v, s);
+        case UGEMM_MODE_BF16X6: return launch_np_conv<3>(g, cd, v, s);
+        case UGEMM_MODE_FP16X3: return launch_np_conv<2, FmtF16>(g, cd, v, s);
+        default: return fail(UGEMM_ERR_UNSUPPORTED, "conv: mode %d is not provided", mode);
+    }
 }
 
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s) {

@@ -74,6 +74,9 @@ int mfma_variant_count(int mode);
 const char *mfma_variant_name(int mode, int v);
 int mfma_pick_variant(const Gemm<float> &g, int mode);
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
+// Convolution in a split form: g.A = weights (g.M x g.K, k-contiguous), g.B
+// unused -- the im2col operand of cd is gathered into planes.
+int launch_mfma_conv(const Gemm<float> &g, const ConvDesc &cd, int mode, int variant, hipStream_t s);
 
 int fp8_variant_count();
 const char *fp8_variant_name(int v);

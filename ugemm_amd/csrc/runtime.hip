@@ -466,7 +466,8 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
     int hcol, wcol;
     int rc = conv_shape(ich, w, h, k, pad, stride, ch, &hcol, &wcol);
     if (rc) return rc;
-    if (mode != UGEMM_MODE_FP32) return fail(UGEMM_ERR_UNSUPPORTED, "conv: mode %d not provided (FP32 only)", mode);
+    if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "conv: unknown mode %d", mode);
+    if (mode == UGEMM_MODE_FP8_BLOCK) return fail(UGEMM_ERR_UNSUPPORTED, "conv: the FP8 form is not provided");
     if (!d_in || !d_weights || !d_out) return fail(UGEMM_ERR_ARG, "conv: NULL buffer");
     const int K = ich * k * k;
     void *ws = nullptr;
@@ -480,8 +481,8 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
         Gemm<float> g{ch, hcol * wcol, K, 1.0f, 0.0f, d_weights, K, nullptr, 0, d_out, (long)hcol * wcol, false, false};
         g.epi = epi;
         const ConvDesc cd{d_in, ktab, h, w, wcol, stride, pad};
-        int v = forced_variant(UGEMM_MODE_FP32);
-        rc = launch_conv_fp32(g, cd, v, s);
+        const int v = forced_variant(mode);
+        rc = mode == UGEMM_MODE_FP32 ? launch_conv_fp32(g, cd, v, s) : launch_mfma_conv(g, cd, mode, v, s);
     }
     const int rc2 = stream_free(ws, s);
     return rc ? rc : rc2;
