@@ -219,7 +219,7 @@ int ugemm_hip_force_variant(int mode, int variant);
  * thread, mode -1 for DGEMM; 0 restores tuning/heuristics. A split call
  * writes alpha = 1 partial products of K chunks to a workspace and adds
  * them in a fixed order (bitwise reproducible). No counterpart in the
- * reference; FP8 and convolutions do not split. */
+ * reference; FP8 and grouped plans do not split. */
 int ugemm_hip_force_splits(int mode, int splits);
 int ugemm_hip_variant_count(int mode);
 const char *ugemm_hip_variant_name(int mode, int variant);

@@ -102,6 +102,29 @@ def test_conv2d_argument_errors():
     assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 1, 0.1, 42, None) == ug.ERR_ARG
 
 
+@pytest.mark.parametrize("mode", [ug.MODE_FP32, ug.MODE_BF16X3])
+@pytest.mark.parametrize("splits", [1, 3, 0])
+def test_conv2d_split_k(mode, splits):
+    """A convolution with few output tiles and a long K (ich * k * k = 4608)
+    split along K (forced 3, or automatic) matches the unsplit bound."""
+    ich, h, w, k, pad, stride, ch = 512, 12, 12, 3, 1, 1, 40
+    g = orc.rng(900 + splits + mode)
+    x, wts, bias = orc.uniform(g, (ich, h, w)), orc.uniform(g, (ch, ich * k * k)), orc.uniform(g, (ch,))
+    hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
+    dx, dw, db = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(wts), hip.DeviceArray.from_host(bias)
+    dout = hip.DeviceArray((ch, hcol * wcol), np.float32)
+    ug.force_splits(mode, splits)
+    try:
+        ug.conv2d_dev(dx, ich, w, h, dw, k, pad, stride, dout, ch, db, act=ug.ACT_LEAKY_RELU, slope=0.1, mode=mode)
+        hip.synchronize()
+    finally:
+        ug.force_splits(mode, 0)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=1, slope=0.1)
+    ok, worst = orc.check_bound(dout.download(), exact, scale,
+                                orc.tolerance(ich * k * k, extra=MODE_EXTRA.get(mode, 0.0)))
+    assert ok, f"mode {mode} splits {splits}: worst/bound {worst:.3g}"
+
+
 @pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1, ug.MODE_BF16X6])
 @pytest.mark.parametrize("shape", [(3, 17, 13, 3, 1, 1, 40), (16, 28, 28, 3, 1, 1, 64), (5, 20, 31, 5, 2, 2, 9),
                                    (1, 9, 9, 1, 0, 1, 300), (32, 14, 14, 3, 1, 1, 300)])

@@ -159,7 +159,7 @@ int launch_cfg(const Gemm<double> &g, hipStream_t s) {
     if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "dgemm: too many tiles (%ld)", nblk);
     const bool vec = ugemm_aligned16(g.A) && ugemm_aligned16(g.B) && (g.lda % 2 == 0) && (g.ldb % 2 == 0);
     int kc = g.K;
-    const int splits = split_count(g.splits, nblk, g.K, 64, &kc);
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc, 2);
     double *ws = nullptr;
     if (splits > 1) {
         const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(double), s);

@@ -884,7 +884,7 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
     const int vec_ab = (vec_ok(g.A, g.lda, FA) ? 1 : 0) | (vec_ok(g.B, g.ldb, FB) ? 2 : 0);
     // Split-K: chunks of whole 32-wide K tiles.
     int kc = Kp;
-    const int splits = split_count(g.splits, (long)tiles_m * tiles_n, Kp, CF::BK, &kc);
+    const int splits = split_count(g.splits, (long)tiles_m * tiles_n, Kp, CF::BK, &kc, CF::NT >= 512 ? 1 : 2);
     float *ws = nullptr;
     if (splits > 1) {
         const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
@@ -1011,7 +1011,7 @@ int run_cfg_conv(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
                            s, amax_a, Mp + Np);
     rc = prep_operand<NP, FMT>(g.A, g.lda, true, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s);
     if (!rc) {
-        constexpr int KSPAN = 512;
+        constexpr int KSPAN = 32;   // short spans: enough blocks for small images
         if (FMT::SCALED)
             hipLaunchKernelGGL(row_amax_im2col, dim3((unsigned)((g.N + 255) / 256), (unsigned)((g.K + KSPAN - 1) / KSPAN)),
                                dim3(256), 0, s, cd, g.N, g.K, KSPAN, amax_b);

@@ -112,10 +112,10 @@ int launch_dgemv(char trans, int M, int N, double alpha, const double *A, int ld
 // [z*kc, min(K, (z+1)*kc)) into workspace slice z (M x N, row length N), and
 // a reduction adds the slices in z order -- a fixed order, so results are
 // bitwise reproducible -- then applies alpha, beta and the epilogue.
-// split_count: the number of splits for a grid of `tiles` tiles (requested
-// > 0: that many, capped so that every split has a K chunk); *kc is set to a
-// multiple of `kalign`.
-int split_count(int requested, long tiles, int K, int kalign, int *kc);
+// split_count: the number of splits for a grid of `tiles` tiles whose kernel
+// wants per_cu resident blocks per CU (requested > 0: that many, capped so
+// that every split has a K chunk); *kc is set to a multiple of `kalign`.
+int split_count(int requested, long tiles, int K, int kalign, int *kc, int per_cu = 1);
 int launch_splitk_reduce_f32(int M, int N, int splits, float alpha, const float *ws, float beta, float *C, long ldc,
                              const Epilogue &epi, hipStream_t s);
 int launch_splitk_reduce_f64(int M, int N, int splits, double alpha, const double *ws, double beta, double *C,

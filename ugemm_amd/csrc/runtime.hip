@@ -480,6 +480,7 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
     if (!rc) {
         Gemm<float> g{ch, hcol * wcol, K, 1.0f, 0.0f, d_weights, K, nullptr, 0, d_out, (long)hcol * wcol, false, false};
         g.epi = epi;
+        g.splits = forced_splits(mode);
         const ConvDesc cd{d_in, ktab, h, w, wcol, stride, pad};
         const int v = forced_variant(mode);
         rc = mode == UGEMM_MODE_FP32 ? launch_conv_fp32(g, cd, v, s) : launch_mfma_conv(g, cd, mode, v, s);

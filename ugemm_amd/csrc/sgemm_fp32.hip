@@ -33,16 +33,27 @@ __global__ __launch_bounds__(CF::NT, CF::MINB) void sgemm_fp32_kernel(int M, int
 // Implicit-GEMM convolution: C (M = output channels x N = output pixels) =
 // alpha * W (M x K, row-major weights) * im2col(in) + beta * C, then the
 // epilogue.  The column matrix is gathered in the tile loader (ConvDesc).
+// Split-K as in sgemm_fp32_kernel: the chunk's rows of the offset table
+// (absolute image offsets) and its weight columns.
 template <class CF, bool TA, int VEC>
 __global__ __launch_bounds__(CF::NT) void conv_fp32_kernel(int M, int N, int K, float alpha,
                                                            const float *__restrict__ A, long lda, float beta,
                                                            float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
-                                                           Epilogue epi, ConvDesc cd) {
+                                                           Epilogue epi, ConvDesc cd, int kc, float *__restrict__ ws) {
     __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    fp32_tile<CF, TA, false, VEC, true>(M, N, K, alpha, A, lda, nullptr, 0, beta, C, ldc, tm * CF::BM, tn * CF::BN,
-                                        lds, true, epi, cd);
+    if (gridDim.y > 1) {
+        const int k0 = blockIdx.y * kc, kz = min(kc, K - k0);
+        ConvDesc cz = cd;
+        cz.ktab = cd.ktab + k0;
+        const float *Az = TA ? A + (long)k0 * lda : A + k0;
+        fp32_tile<CF, TA, false, VEC, true>(M, N, kz, 1.0f, Az, lda, nullptr, 0, 0.0f, ws + (long)blockIdx.y * M * N,
+                                            N, tm * CF::BM, tn * CF::BN, lds, true, Epilogue(), cz);
+    } else {
+        fp32_tile<CF, TA, false, VEC, true>(M, N, K, alpha, A, lda, nullptr, 0, beta, C, ldc, tm * CF::BM,
+                                            tn * CF::BN, lds, true, epi, cd);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -68,7 +79,7 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     // (split-K chunk offsets are multiples of 64 columns, which keeps that).
     const bool vec = ugemm_aligned16(g.A) && ugemm_aligned16(g.B) && (g.lda % 4 == 0) && (g.ldb % 4 == 0);
     int kc = g.K;
-    const int splits = split_count(g.splits, nblk, g.K, 64, &kc);
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc, 2);
     float *ws = nullptr;
     if (splits > 1) {
         const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
@@ -109,20 +120,33 @@ int launch_conv_cfg(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
     const long nblk = (long)tiles_m * tiles_n;
     if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "conv: too many tiles (%ld)", nblk);
     const bool vec = ugemm_aligned16(g.A) && (g.lda % 4 == 0);
-    dim3 grid((unsigned)nblk), block(CF::NT);
+    int kc = g.K;
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc, 2);
+    float *ws = nullptr;
+    if (splits > 1) {
+        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
+        if (rc) return rc;
+    }
+    dim3 grid((unsigned)nblk, (unsigned)splits), block(CF::NT);
     timing_begin(s);
     if (vec && !g.ta)
         hipLaunchKernelGGL((conv_fp32_kernel<CF, false, 4>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
-                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd, kc, ws);
     else if (!g.ta)
         hipLaunchKernelGGL((conv_fp32_kernel<CF, false, 1>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
-                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd, kc, ws);
     else
         hipLaunchKernelGGL((conv_fp32_kernel<CF, true, 1>), grid, block, 0, s, g.M, g.N, g.K, g.alpha, g.A, g.lda,
-                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd);
+                           g.beta, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd, kc, ws);
     timing_end(s);
-    UGEMM_HIP_TRY(hipGetLastError());
-    return UGEMM_OK;
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "conv launch");
+    if (splits > 1) {
+        if (!rc) rc = launch_splitk_reduce_f32(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, g.epi, s);
+        const int rc2 = stream_free(ws, s);
+        if (!rc) rc = rc2;
+    }
+    return rc;
 }
 
 }  // namespace

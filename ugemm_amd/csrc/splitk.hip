@@ -39,14 +39,15 @@ int reduce(int M, int N, int S, T alpha, const T *ws, T beta, T *C, long ldc, co
 
 }  // namespace
 
-int split_count(int requested, long tiles, int K, int kalign, int *kc) {
+int split_count(int requested, long tiles, int K, int kalign, int *kc, int per_cu) {
     int S = requested;
     if (S <= 0) {
-        // Auto: split only when the grid leaves more than half the CUs idle,
-        // enough to give every CU a tile, with at least 512 K per split.
-        const long cus = device_cu_count();
+        // Auto: split only when the grid fills at most half of the wanted
+        // per_cu blocks on every CU, enough to reach that, with at least
+        // 512 K per split.
+        const long want = (long)per_cu * device_cu_count();
         S = 1;
-        if (tiles * 2 <= cus) S = (int)std::min<long>({cus / std::max<long>(1, tiles), (long)K / 512, 16L});
+        if (tiles * 2 <= want) S = (int)std::min<long>({want / std::max<long>(1, tiles), (long)K / 512, 16L});
         S = std::max(S, 1);
     }
     int chunk = (K + S - 1) / S;
