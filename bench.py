@@ -394,6 +394,48 @@ def blas12(stream):
     return res
 
 
+def split_k(stream):
+    """Few tiles, long K: automatic split-K against one split (FP32, BF16x3,
+    FP64), with the split result's error bound checked on a row sample."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D06))
+    res = {}
+    for (M, N, K) in ((256, 256, 65536), (1024, 1024, 16384)):
+        A, B = g.standard_normal((M, K), dtype=np.float32), g.standard_normal((K, N), dtype=np.float32)
+        rows = np.arange(0, M, max(1, M // 16))
+        exact = A[rows].astype(np.float64) @ B.astype(np.float64)
+        scale = np.abs(A[rows].astype(np.float64)) @ np.abs(B.astype(np.float64))
+        key = f"{M}x{N}x{K}"
+        res[key] = {}
+        for form in ("fp32", "bf16x3", "fp64"):
+            if form == "fp64":
+                dA, dB = hip.DeviceArray.from_host(A.astype(np.float64)), hip.DeviceArray.from_host(B.astype(np.float64))
+                dC, mid = hip.DeviceArray((M, N), np.float64), ug.MODE_DGEMM
+
+                def call():
+                    ug.dgemm_dev("R", "N", "N", M, N, K, 1.0, dA, K, dB, N, 0.0, dC, N, stream=stream)
+            else:
+                dA, dB = hip.DeviceArray.from_host(A), hip.DeviceArray.from_host(B)
+                dC, mid = hip.DeviceArray((M, N), np.float32), ug.MODES[form]
+
+                def call():
+                    ug.sgemm_dev("R", "N", "N", M, N, K, 1.0, dA, K, dB, N, 0.0, dC, N, mode=mid, stream=stream)
+            r = {}
+            for label, sp in (("one_split", 1), ("auto", 0)):
+                ug.force_splits(mid, sp)
+                try:
+                    ms, _, _ = timed(call, 10, 2, stream)
+                finally:
+                    ug.force_splits(mid, 0)
+                r[label + "_tflops"] = round(2.0 * M * N * K / (ms * 1e-3) / 1e12, 3)
+            got = dC.download()[rows].astype(np.float64)
+            r["auto_max_err_over_scale"] = float(np.max(np.abs(got - exact) / np.maximum(scale, 1e-300)))
+            res[key][form] = r
+            log(f"[split_k] {key} {form}: one split {r['one_split_tflops']:.1f}, auto {r['auto_tflops']:.1f} TFLOP/s")
+            for d in (dA, dB, dC):
+                d.free()
+    return res
+
+
 def host_path():
     """Host-pointer calls, copies included (the sgemm_ocl-style use of
     sgemm_ocl2.h:166-219): sgemm_test.c's own workload (M=1023, N=1000,
@@ -563,6 +605,7 @@ def main():
         fp32_tf = res.get("fp32", {}).get("tflops")
         line["config4_batched"] = config4(stream, fp32_tf)
         line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
+        line["split_k"] = split_k(stream)
         line["blas12"] = blas12(stream)
         line["host_pointer_path"] = host_path()
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
