@@ -254,7 +254,7 @@ def config4(stream, fp32_single_tf):
     ms_loop = e0.elapsed_ms(e1) / loop_reps
     plan.destroy()
     tf_group = flop / (ms_group * 1e-3) / 1e12
-    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.variants(ug.MODE_FP32)[tuned],
+    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
          "grouped_graph_ms": round(ms_group, 4), "grouped_tflops": round(tf_group, 3),
          "gemms_per_s": round(count / (ms_group * 1e-3), 1),
          "loop_plain_ms": round(ms_loop, 4), "loop_plain_tflops": round(flop / (ms_loop * 1e-3) / 1e12, 3),
@@ -332,7 +332,7 @@ def config4_conv(stream, gemm_grouped_tf):
         worst = max(worst, float(np.max(np.abs(got - ref) / (2 * (c * 9 + 1) * 2.0 ** -24 * sc))))
     plan.destroy()
     tf = flop / (ms * 1e-3) / 1e12
-    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.variants(ug.MODE_FP32)[tuned],
+    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
          "grouped_graph_ms": round(ms, 4), "grouped_tflops": round(tf, 3),
          "vs_grouped_gemm_same_shapes": round(tf / gemm_grouped_tf, 4) if gemm_grouped_tf else None,
          "checked_problems_worst_err_over_bound": worst,

@@ -167,6 +167,13 @@ int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
 /* One-shot convenience: plan, run (no capture), destroy. */
 int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode,
                             void *stream);
+/* Tile variants of the grouped kernel (no counterpart in the reference):
+ * 0-3 one tile shape for every problem, 4 a tile shape per problem (128 x
+ * 128 or 64 x 128 by the padding of its rows).  ugemm_hip_tune_grouped
+ * times them; ugemm_hip_force_variant(UGEMM_MODE_FP32, v) with v < the
+ * count pins one for the plans created next on the thread. */
+int ugemm_hip_grouped_variant_count(void);
+const char *ugemm_hip_grouped_variant_name(int variant);
 
 /* ------------------------------------------------------------------ */
 /* 5. BLAS-1/2 helpers (HBM-bound kernels).                           */

@@ -110,7 +110,37 @@ def test_grouped_tuning_keeps_results():
     s.destroy()
 
 
-# A forced single-GEMM variant the grouped kernel lacks (4, 5) is ignored.
+def test_grouped_mixed_tile_shapes():
+    """Variant 4 gives each problem its own tile shape (128 x 128 or 64 x 128
+    by the padding of its rows): one plan mixing both, every layout, checked
+    per problem, replayed from a graph twice (bitwise equal)."""
+    assert ug.grouped_variants()[4].startswith("fp32_mixed")
+    assert len(ug.grouped_variants()) == 5
+    cases = []
+    for i, M in enumerate((32, 64, 96, 128, 160, 192, 224, 256, 300, 352, 512, 33)):
+        ta, tb = ("N", "T")[i % 2], ("N", "T")[(i // 2) % 2]
+        cases.append(DevCase(Case("R", ta, tb, M, 49 + 37 * i, 144 * (1 + i % 4), 1.5, 0.5, seed=400 + i, pad=i % 3)))
+    ug.force_variant(ug.MODE_FP32, 4)
+    try:
+        plan = ug.GroupedPlan([d.problem() for d in cases], capture=True)
+    finally:
+        ug.force_variant(ug.MODE_FP32, -1)
+    outs = []
+    for _ in range(2):
+        for d in cases:
+            d.reset()
+        plan.run(None)
+        hip.synchronize()
+        outs.append([d.dC.download() for d in cases])
+        for d in cases:
+            ok, worst = d.check()
+            assert ok, f"M={d.case.M}: {worst:.3g}"
+    plan.destroy()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# A forced single-GEMM variant the grouped kernel lacks (5) is ignored.
 @pytest.mark.parametrize("variant", range(6))
 def test_grouped_every_variant(variant):
     ug.force_variant(ug.MODE_FP32, variant)

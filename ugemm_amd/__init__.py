@@ -87,6 +87,8 @@ def load(path=None):
         "ugemm_hip_status_string": (cs, [i]),
         "ugemm_hip_tune_sgemm": (i, [c, c, i, i, i, i]),
         "ugemm_hip_tune_grouped": (i, [vp]),
+        "ugemm_hip_grouped_variant_count": (i, []),
+        "ugemm_hip_grouped_variant_name": (cs, [i]),
         "ugemm_hip_tune_load": (i, [cs]),
         "ugemm_hip_tune_save": (i, [cs]),
         "ugemm_hip_tune_lookup": (i, [c, c, i, i, i, i]),
@@ -251,6 +253,12 @@ def timing_collect():
 def variants(mode):
     L = load()
     return [L.ugemm_hip_variant_name(mode, v).decode() for v in range(L.ugemm_hip_variant_count(mode))]
+
+
+def grouped_variants():
+    """Names of the grouped kernel's tile variants (indices of ugemm_hip_tune_grouped)."""
+    L = load()
+    return [L.ugemm_hip_grouped_variant_name(v).decode() for v in range(L.ugemm_hip_grouped_variant_count())]
 
 
 def force_variant(mode, variant):

@@ -31,6 +31,7 @@
 #include "fp32_tile.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -50,7 +51,7 @@ struct DevProblem {
 };
 
 struct DevTile {
-    int problem, m0, n0, pad;
+    int problem, m0, n0, shape;   // shape: which tile of a mixed plan (0: the larger, 1: the smaller)
 };
 
 constexpr int kQueues = 8;   // one per XCD
@@ -59,14 +60,52 @@ __global__ void grouped_reset_heads(int *heads) {
     if (threadIdx.x < kQueues) heads[threadIdx.x] = 0;
 }
 
+// One tile of problem p at (m0, n0), dispatched on the problem's layout.
+template <class CF, bool CONV>
+__device__ __forceinline__ void grouped_tile(const DevProblem &p, int m0, int n0, float *lds) {
+    const bool aligned = (p.layout & 4) != 0;
+    if constexpr (CONV) {
+        fp32_tile<CF, false, false, 4, true>(p.M, p.N, p.K, p.alpha, p.A, p.lda, nullptr, 0, p.beta, p.C, p.ldc, m0,
+                                             n0, lds, aligned, p.epi, p.cd);
+        return;
+    }
+    switch (p.layout & 3) {
+        case 0:
+            fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
+                                           lds, aligned);
+            break;
+        case 1:
+            fp32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
+                                          lds, aligned);
+            break;
+        case 2:
+            fp32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
+                                          lds, aligned);
+            break;
+        default:
+            fp32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
+                                         lds, aligned);
+            break;
+    }
+}
+
+template <class CF>
+constexpr int lds_floats() {
+    return 2 * CF::BK * (CF::LDA + CF::LDB);
+}
+
 // At least 3 waves per SIMD (<= 168 VGPRs): the layout switch would
 // otherwise push the 128 x 128 variant just past that limit.
 // CONV: a plan of convolutions (every problem has layout bit 3), else of GEMMs.
-template <class CF, bool CONV>
+// CS: the tile of a mixed plan whose DevTile::shape is 1 (CF for shape 0);
+// the same as CF for a single-shape plan.  Both shapes have CF::NT threads.
+template <class CF, bool CONV, class CS = CF>
 __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) void grouped_fp32_kernel(
     const DevProblem *__restrict__ probs, const DevTile *__restrict__ tiles, const int *__restrict__ q_begin,
     int *heads) {
-    __shared__ __attribute__((aligned(16))) float lds[2 * CF::BK * (CF::LDA + CF::LDB)];
+    static_assert(CS::NT == CF::NT, "the shapes of a mixed plan share the workgroup size");
+    constexpr int LDS_F = lds_floats<CF>() > lds_floats<CS>() ? lds_floats<CF>() : lds_floats<CS>();
+    __shared__ __attribute__((aligned(16))) float lds[LDS_F];
     __shared__ int s_tile;
     int q = blockIdx.x % kQueues;
     for (int empty = 0; empty < kQueues;) {
@@ -85,31 +124,8 @@ __global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) voi
         }
         const DevTile tl = tiles[i];
         const DevProblem p = probs[tl.problem];
-        const bool aligned = (p.layout & 4) != 0;
-        if constexpr (CONV) {
-            fp32_tile<CF, false, false, 4, true>(p.M, p.N, p.K, p.alpha, p.A, p.lda, nullptr, 0, p.beta, p.C, p.ldc,
-                                                 tl.m0, tl.n0, lds, aligned, p.epi, p.cd);
-            __syncthreads();
-            continue;
-        }
-        switch (p.layout & 3) {
-            case 0:
-                fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                               tl.m0, tl.n0, lds, aligned);
-                break;
-            case 1:
-                fp32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                              tl.m0, tl.n0, lds, aligned);
-                break;
-            case 2:
-                fp32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                              tl.m0, tl.n0, lds, aligned);
-                break;
-            default:
-                fp32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc,
-                                             tl.m0, tl.n0, lds, aligned);
-                break;
-        }
+        if (std::is_same<CF, CS>::value || tl.shape == 0) grouped_tile<CF, CONV>(p, tl.m0, tl.n0, lds);
+        else grouped_tile<CS, CONV>(p, tl.m0, tl.n0, lds);
         __syncthreads();   // the next tile's first LDS stage overwrites this one's
     }
 }
@@ -118,18 +134,22 @@ using G0 = Fp32Cfg<128, 128, 16, 8, 8>;
 using G1 = Fp32Cfg<64, 64, 16, 4, 4>;
 using G2 = Fp32Cfg<128, 64, 16, 8, 4>;
 using G3 = Fp32Cfg<64, 128, 16, 4, 8>;
-constexpr int kNumVariants = 4;
+// Variant 4 ("mixed"): every problem gets the 128 x 128 tile (G0) or the
+// 64 x 128 tile (G3), whichever covers its rows with less padding (the
+// larger, more efficient tile when it pads at most 10% more).
+constexpr int kNumVariants = 5, kMixed = 4;
 
 struct VariantOps {
-    int bm, bn, bk;
+    int bm, bn, bk;          // shape 0
+    int bm1, bn1;            // shape 1 (mixed plans; equal to shape 0 otherwise)
     const void *kern, *kern_conv;
     int nt;
 };
 
-template <class CF>
+template <class CF, class CS = CF>
 VariantOps ops_of() {
-    return VariantOps{CF::BM, CF::BN, CF::BK, (const void *)grouped_fp32_kernel<CF, false>,
-                      (const void *)grouped_fp32_kernel<CF, true>, CF::NT};
+    return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
+                      (const void *)grouped_fp32_kernel<CF, true, CS>, CF::NT};
 }
 
 VariantOps variant_ops(int v) {
@@ -137,11 +157,25 @@ VariantOps variant_ops(int v) {
         case 0: return ops_of<G0>();
         case 1: return ops_of<G1>();
         case 2: return ops_of<G2>();
-        default: return ops_of<G3>();
+        case 3: return ops_of<G3>();
+        default: return ops_of<G0, G3>();
     }
 }
 
+const char *kGroupedNames[] = {"fp32_128x128x16_t8x8", "fp32_64x64x16_t4x4", "fp32_128x64x16_t8x4",
+                               "fp32_64x128x16_t4x8", "fp32_mixed_128x128_or_64x128_per_problem"};
+
+// Tile shape of problem rows M in a mixed plan: 0 (BM0 rows) unless its
+// padded rows exceed those of shape 1 (BM1 rows) by more than 10%.
+int mixed_shape(int M, int bm0, int bm1) {
+    const long r0 = (long)(M + bm0 - 1) / bm0 * bm0, r1 = (long)(M + bm1 - 1) / bm1 * bm1;
+    return (r0 * 10 <= r1 * 11) ? 0 : 1;
+}
+
 }  // namespace
+
+int grouped_variant_count() { return kNumVariants; }
+
 }  // namespace ugemm
 
 using namespace ugemm;
@@ -209,15 +243,18 @@ int build_schedule(ugemm_grouped_plan *p, int v) {
         const DevProblem &q = p->probs[i];
         if (q.M == 0 || q.N == 0) continue;
         if (q.K == 0 && q.beta == 1.0f) continue;   // nothing to do
-        const long cost = (q.K + ops.bk - 1) / ops.bk + 2;
+        // Shape of this problem's tiles; cost ~ k-steps x tile rows.
+        const int shape = (v == kMixed) ? mixed_shape(q.M, ops.bm, ops.bm1) : 0;
+        const int bm = shape ? ops.bm1 : ops.bm, bn = shape ? ops.bn1 : ops.bn;
+        const long cost = ((q.K + ops.bk - 1) / ops.bk + 2) * bm;
         // Grouped order: GROUP_M tile-rows at a time, walked column by column.
         constexpr int GROUP_M = 8;
-        const int tiles_m = (q.M + ops.bm - 1) / ops.bm, tiles_n = (q.N + ops.bn - 1) / ops.bn;
+        const int tiles_m = (q.M + bm - 1) / bm, tiles_n = (q.N + bn - 1) / bn;
         for (int g0 = 0; g0 < tiles_m; g0 += GROUP_M) {
             const int g1 = std::min(tiles_m, g0 + GROUP_M);
             for (int tn = 0; tn < tiles_n; ++tn)
                 for (int tm = g0; tm < g1; ++tm)
-                    tiles.push_back(HostTile{DevTile{i, tm * ops.bm, tn * ops.bn, 0}, cost});
+                    tiles.push_back(HostTile{DevTile{i, tm * bm, tn * bn, shape}, cost});
         }
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile &a, const HostTile &b) { return a.cost > b.cost; });
@@ -461,6 +498,10 @@ int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode
 // Times each tile variant on the plan's own problems (C is overwritten, as by
 // a run) and keeps the fastest; the choice is remembered for plans with the
 // same problem count and work bucket.  Returns the chosen variant.
+int ugemm_hip_grouped_variant_count(void) { return kNumVariants; }
+
+const char *ugemm_hip_grouped_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kGroupedNames[v] : nullptr; }
+
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (!p) return fail(UGEMM_ERR_ARG, "grouped: NULL plan");
     if (p->grid == 0) return p->variant;

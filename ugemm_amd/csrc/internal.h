@@ -121,8 +121,11 @@ int launch_splitk_reduce_f32(int M, int N, int splits, float alpha, const float 
 int launch_splitk_reduce_f64(int M, int N, int splits, double alpha, const double *ws, double beta, double *C,
                              long ldc, hipStream_t s);
 
+// Tile variants of the grouped kernel (grouped.hip).
+int grouped_variant_count();
+
 // ---- tuning (tune.hip) ----
-// Tuning-table key of grouped plans (variants = the 4 FP32 tile shapes).
+// Tuning-table key of grouped plans (variants = grouped_variant_count()).
 constexpr int TUNE_KEY_GROUPED = 16;
 int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits = nullptr);   // -1 if none
 int forced_variant(int mode);                                         // -1 if none
