@@ -253,6 +253,29 @@ def config4(stream, fp32_single_tf):
     e1.synchronize()
     ms_loop = e0.elapsed_ms(e1) / loop_reps
     plan.destroy()
+    # The same batch in the BF16x3 form: one preparation launch for every
+    # operand, one split-core block per tile (captured, tuned); its own
+    # bound (2^-14 per product) on the same problems.
+    plan3 = ug.GroupedPlan(probs, mode=ug.MODE_BF16X3, capture=True)
+    tuned3 = plan3.tune()
+    for _ in range(2):
+        plan3.run(stream)
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        plan3.run(stream)
+    e1.record(stream)
+    e1.synchronize()
+    ms_group3 = e0.elapsed_ms(e1) / reps
+    worst3 = 0.0
+    for i in np.argsort(Ms * Ns * Ks)[:6]:
+        M, N, K = int(Ms[i]), int(Ns[i]), int(Ks[i])
+        A = pa[a_off[i]:a_off[i] + M * K].reshape(M, K).astype(np.float64)
+        B = pb[b_off[i]:b_off[i] + K * N].reshape(K, N).astype(np.float64)
+        got = dC.download()[c_off[i]:c_off[i] + M * N].reshape(M, N).astype(np.float64)
+        bound = (2 * (K + 1) * 2.0 ** -24 + 2.0 ** -14) * (np.abs(A) @ np.abs(B))
+        worst3 = max(worst3, float(np.max(np.abs(got - A @ B) / bound)))
+    plan3.destroy()
     tf_group = flop / (ms_group * 1e-3) / 1e12
     r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
          "grouped_graph_ms": round(ms_group, 4), "grouped_tflops": round(tf_group, 3),
@@ -260,9 +283,14 @@ def config4(stream, fp32_single_tf):
          "loop_plain_ms": round(ms_loop, 4), "loop_plain_tflops": round(flop / (ms_loop * 1e-3) / 1e12, 3),
          "speedup_vs_loop": round(ms_loop / ms_group, 3),
          "frac_of_single_large_gemm_rate": round(tf_group / fp32_single_tf, 4) if fp32_single_tf else None,
-         "checked_problems_worst_err_over_bound": worst}
+         "checked_problems_worst_err_over_bound": worst,
+         "bf16x3_grouped": {"split_core_variant": ug.variants(ug.MODE_BF16X3)[tuned3],
+                            "grouped_graph_ms": round(ms_group3, 4),
+                            "grouped_tflops": round(flop / (ms_group3 * 1e-3) / 1e12, 3),
+                            "checked_problems_worst_err_over_bound": worst3}}
     log(f"[config4] grouped {tf_group:.1f} TFLOP/s ({ms_group:.3f} ms), loop {r['loop_plain_tflops']:.1f} TFLOP/s, "
-        f"{r['frac_of_single_large_gemm_rate']} of single-GEMM rate")
+        f"{r['frac_of_single_large_gemm_rate']} of single-GEMM rate; bf16x3 grouped "
+        f"{r['bf16x3_grouped']['grouped_tflops']:.1f} TFLOP/s")
     for d in (dpa, dpb, dC):
         d.free()
     e0.destroy()

@@ -140,6 +140,53 @@ def test_grouped_mixed_tile_shapes():
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+SPLIT_GROUPED = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6]
+
+
+@pytest.mark.parametrize("mode", SPLIT_GROUPED)
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3])
+def test_grouped_split_forms(mode, variant):
+    """Grouped plans in the BF16 split forms: every problem's planes from one
+    preparation launch, one split-core block per tile; uneven im2col-shaped
+    problems of every layout (and the tiny / K = 0 / alpha = 0 edge
+    problems) meet the form's bound (helpers.MODE_EXTRA); -1 = default tile."""
+    cases = [DevCase(c) for c in uneven_cases(14, 50 + variant + 7 * mode)]
+    ug.force_variant(mode, variant)
+    try:
+        plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=False)
+    finally:
+        ug.force_variant(mode, -1)
+    plan.run(None)
+    hip.synchronize()
+    for d in cases:
+        ok, worst = d.case.judge(d.dC.download(), mode=mode)
+        assert ok, f"mode {mode} variant {variant} {d.case.M}x{d.case.N}x{d.case.K}: {worst:.3g}"
+    plan.destroy()
+
+
+def test_grouped_split_form_graph_replays_and_tuning():
+    """A captured BF16x3 plan replays bitwise identically; tuning picks one
+    of the four split-core tiles and leaves results within the bound."""
+    mode = ug.MODE_BF16X3
+    cases = [DevCase(c) for c in uneven_cases(20, 91, max_cin=128, sizes=(7, 14, 28, 56))]
+    plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
+    v = plan.tune()
+    assert 0 <= v < 4
+    outs = []
+    for _ in range(2):
+        for d in cases:
+            d.reset()
+        plan.run(None)
+        hip.synchronize()
+        outs.append([d.dC.download() for d in cases])
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for d, C in zip(cases, outs[0]):
+        ok, worst = d.case.judge(C, mode=mode)
+        assert ok, worst
+    plan.destroy()
+
+
 # A forced single-GEMM variant the grouped kernel lacks (5) is ignored.
 @pytest.mark.parametrize("variant", range(6))
 def test_grouped_every_variant(variant):
@@ -165,7 +212,10 @@ def test_grouped_argument_errors():
     assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP32, None) == ug.ERR_ARG
     ok = ug.Problem(b"N", b"N", 4, 4, 4, 1.0, 0.0, 16, 4, 16, 4, 16, 4)
     arr = (ug.Problem * 1)(ok)
-    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_BF16X3, None) == ug.ERR_UNSUPPORTED
+    # Forms without a grouped path are refused before any work (the pointers
+    # here are not device memory).
+    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP16X3, None) == ug.ERR_UNSUPPORTED
+    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP8_BLOCK, None) == ug.ERR_UNSUPPORTED
     assert L.ugemm_hip_sgemm_grouped(0, None, ug.MODE_FP32, None) == ug.OK
 
 

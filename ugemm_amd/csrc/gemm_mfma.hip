@@ -136,45 +136,51 @@ __device__ __forceinline__ long blocked_offset(int r, int k, int rows_pad) {
 }
 
 // Source element (r, k) of the operand: KCONT ? X[r*ld + k] : X[k*ld + r];
-// zeros outside rows x K.  KCONT version: one thread per 8 k of one row.
+// zeros outside rows x K.  KCONT version: one thread per 8 k of one row
+// (chunk f of the rows_pad x k_pad/8 chunks).
 // FMT::SCALED: row r is multiplied by 2^-row_exp(amax[r]) before splitting.
+template <int NP, class FMT>
+__device__ __forceinline__ void prep_kcont_chunk(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                 typename FMT::E *__restrict__ out, long plane_stride, int rows_pad,
+                                                 int k_pad, const unsigned *__restrict__ amax, long f) {
+    const long chunks_per_row = k_pad / 8;
+    const int r = (int)(f / chunks_per_row);
+    const int k = (int)(f - (long)r * chunks_per_row) * 8;
+    float x[8];
+    const float *p = X + (long)r * ld + k;
+    if (r < rows && k + 8 <= K && vec) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        const float4 b = *reinterpret_cast<const float4 *>(p + 4);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = (r < rows && k + j < K) ? p[j] : 0.0f;
+    }
+    if constexpr (FMT::SCALED) {
+        const int e = -row_exp(amax[r]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
+    }
+    write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
+}
+
 template <int NP, class FMT>
 __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict__ X, long ld, int rows, int K,
                                                         bool vec, typename FMT::E *__restrict__ out,
                                                         long plane_stride, int rows_pad, int k_pad,
                                                         const unsigned *__restrict__ amax) {
-    const long chunks_per_row = k_pad / 8;
-    const long total = (long)rows_pad * chunks_per_row;
-    for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(f / chunks_per_row);
-        const int k = (int)(f - (long)r * chunks_per_row) * 8;
-        float x[8];
-        const float *p = X + (long)r * ld + k;
-        if (r < rows && k + 8 <= K && vec) {
-            const float4 a = *reinterpret_cast<const float4 *>(p);
-            const float4 b = *reinterpret_cast<const float4 *>(p + 4);
-            x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = (r < rows && k + j < K) ? p[j] : 0.0f;
-        }
-        if constexpr (FMT::SCALED) {
-            const int e = -row_exp(amax[r]);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
-        }
-        write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
-    }
+    const long total = (long)rows_pad * (k_pad / 8);
+    for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (long)gridDim.x * blockDim.x)
+        prep_kcont_chunk<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, f);
 }
 
-// Row-contiguous source (X[k*ld + r]): 64 x 64 tiles transposed through LDS.
+// Row-contiguous source (X[k*ld + r]): the 64 x 64 tile (rows r0.., k k0..)
+// transposed through LDS (tile[64][65], 256 threads).
 template <int NP, class FMT>
-__global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict__ X, long ld, int rows, int K,
-                                                        bool vec, typename FMT::E *__restrict__ out,
-                                                        long plane_stride, int rows_pad, int k_pad,
-                                                        const unsigned *__restrict__ amax) {
-    __shared__ float tile[64][65];
-    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+__device__ __forceinline__ void prep_rcont_tile(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                typename FMT::E *__restrict__ out, long plane_stride, int rows_pad,
+                                                int k_pad, const unsigned *__restrict__ amax, int r0, int k0,
+                                                float (*tile)[65]) {
     const int t = threadIdx.x;
     // Read: 64 k-lines of 64 r each, as 16 float4 per line (coalesced along r).
 #pragma unroll
@@ -211,6 +217,40 @@ __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict_
             for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
         }
         write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x);
+    }
+}
+
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                        bool vec, typename FMT::E *__restrict__ out,
+                                                        long plane_stride, int rows_pad, int k_pad,
+                                                        const unsigned *__restrict__ amax) {
+    __shared__ float tile[64][65];
+    prep_rcont_tile<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, blockIdx.x * 64,
+                             blockIdx.y * 64, tile);
+}
+
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restrict__ jobs, int njobs) {
+    __shared__ float tile[64][65];
+    // The job of this block: the last one whose first block is <= blockIdx.x.
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].block0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PrepJob j = jobs[lo];
+    const int lb = (int)blockIdx.x - j.block0;
+    typename FMT::E *out = static_cast<typename FMT::E *>(j.out);
+    if (j.kcont) {
+        const long f = (long)lb * 256 + threadIdx.x;
+        if (f < (long)j.rows_pad * (j.k_pad / 8))
+            prep_kcont_chunk<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
+                                      nullptr, f);
+    } else {
+        prep_rcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
+                                 nullptr, (lb % j.nbx) * 64, (lb / j.nbx) * 64, tile);
     }
 }
 
@@ -436,27 +476,43 @@ enum { FUSE_NONE = 0, FUSE_KCONT = 1, FUSE_RCONT = 2 };
 //     t / (rows/2)), coalesced along r, and splits one 8-k chunk per row.
 // Either way each split chunk lands where the plane copy would have put it
 // (zeros beyond M, N, K).
-template <class CF, int FA, int FB>
-__global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, const typename CF::FMT::E *__restrict__ Ap,
-                                                       long a_plane, int Mp, const typename CF::FMT::E *__restrict__ Bp,
-                                                       long b_plane, int Np, float alpha, float beta,
-                                                       float *__restrict__ C, long ldc, int tiles_m, int tiles_n,
+// GR (grouped plans): block b computes tile gtiles[b] of problem
+// gprobs[gtiles[b].problem], whose planes, sizes, alpha, beta and C replace
+// the corresponding arguments (no split K, no epilogue, no fused operand).
+template <class CF, int FA, int FB, bool GR = false>
+__global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_, const typename CF::FMT::E *__restrict__ Ap_,
+                                                       long a_plane_, int Mp_, const typename CF::FMT::E *__restrict__ Bp_,
+                                                       long b_plane_, int Np_, float alpha_, float beta_,
+                                                       float *__restrict__ C_, long ldc_, int tiles_m, int tiles_n,
                                                        const unsigned *__restrict__ amax_a,
                                                        const unsigned *__restrict__ amax_b, Epilogue epi,
                                                        const float *__restrict__ Af, long lda,
                                                        const float *__restrict__ Bf, long ldb, int K, int vec_ab,
-                                                       int kt_per, float *__restrict__ ws) {
+                                                       int kt_per, float *__restrict__ ws,
+                                                       const GroupedSplitProblem *__restrict__ gprobs,
+                                                       const GroupedSplitTile *__restrict__ gtiles) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
+    static_assert(!GR || (FA == 0 && FB == 0 && !FMT::SCALED), "grouped plans stream prepared, unscaled planes");
+    const GroupedSplitTile gt = GR ? gtiles[blockIdx.x] : GroupedSplitTile{0, 0, 0, 0};
+    const GroupedSplitProblem *gp = GR ? gprobs + gt.problem : nullptr;
+    const int M = GR ? gp->M : M_, N = GR ? gp->N : N_, Kp = GR ? gp->Kp : Kp_;
+    const int Mp = GR ? gp->Mp : Mp_, Np = GR ? gp->Np : Np_;
+    const E *__restrict__ Ap = GR ? static_cast<const E *>(gp->Ap) : Ap_;
+    const E *__restrict__ Bp = GR ? static_cast<const E *>(gp->Bp) : Bp_;
+    const long a_plane = GR ? gp->a_plane : a_plane_, b_plane = GR ? gp->b_plane : b_plane_;
+    const float alpha = GR ? gp->alpha : alpha_, beta = GR ? gp->beta : beta_;
+    float *__restrict__ C = GR ? gp->C : C_;
+    const long ldc = GR ? gp->ldc : ldc_;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
     constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     E *lds = reinterpret_cast<E *>(smem_raw);
 
-    int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
+    int tm = 0, tn = 0;
+    if (!GR) tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = GR ? gt.m0 : tm * BM, n0 = GR ? gt.n0 : tn * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w / CF::WGN, wn = w % CF::WGN;
 
@@ -710,7 +766,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M, int N, int Kp, co
     // 256 accumulators per lane there is no room for the batch (it spills):
     // those variants interleave.
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
-    const bool has_epi = !part && epi.active();
+    const bool has_epi = !GR && !part && epi.active();
     constexpr bool BATCH = FM * FN * 4 < 256;
     constexpr int JB = !BATCH ? 1 : (CF::NT >= 1024 ? 1 : FN);
 #pragma unroll
@@ -893,7 +949,8 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
     timing_begin(s);
     hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(CF::NT), CF::LDS_BYTES, s,
                        g.M, g.N, Kp, Ap, a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n,
-                       amax_a, amax_b, g.epi, g.A, g.lda, g.B, g.ldb, g.K, vec_ab, kc / CF::BK, ws);
+                       amax_a, amax_b, g.epi, g.A, g.lda, g.B, g.ldb, g.K, vec_ab, kc / CF::BK, ws,
+                       (const GroupedSplitProblem *)nullptr, (const GroupedSplitTile *)nullptr);
     timing_end(s);
     hipError_t e = hipGetLastError();
     int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "split core launch");
@@ -1093,6 +1150,78 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= cus / 2) return 1;
     return 3;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped split-form plans (grouped.hip): the 4-wave variants V0-V3, whose
+// tiles suit uneven batches; BF16 forms only (the FP16 form's row scales
+// are not prepared by the grouped preparation kernel).
+namespace {
+template <class CF>
+int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
+                 const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
+    using FMT = typename CF::FMT;
+    hipLaunchKernelGGL((grouped_split_prep<CF::NP, FMT>), dim3((unsigned)prep_blocks), dim3(256), 0, s, jobs, njobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "grouped split-form preparation");
+    auto kern = split_nt_core<CF, FUSE_NONE, FUSE_NONE, true>;
+    static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
+    if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
+    timing_begin(s);
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(CF::NT), CF::LDS_BYTES, s, 0, 0, 0,
+                       (const typename FMT::E *)nullptr, 0L, 0, (const typename FMT::E *)nullptr, 0L, 0, 0.0f, 0.0f,
+                       (float *)nullptr, 0L, 0, 0, (const unsigned *)nullptr, (const unsigned *)nullptr, Epilogue(),
+                       (const float *)nullptr, 0L, (const float *)nullptr, 0L, 0, 0, 0, (float *)nullptr, probs, tiles);
+    timing_end(s);
+    e = hipGetLastError();
+    return e == hipSuccess ? UGEMM_OK : hip_fail(e, "grouped split-form core");
+}
+
+template <int NP>
+int grouped_np(int variant, const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
+               const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
+    using T = Table<NP>;
+    switch (variant) {
+        case 0: return grouped_core<typename T::V0>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 1: return grouped_core<typename T::V1>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 2: return grouped_core<typename T::V2>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 3: return grouped_core<typename T::V3>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        default: return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
+    }
+}
+
+template <int NP>
+void tile_of(int variant, int *bm, int *bn) {
+    using T = Table<NP>;
+    switch (variant) {
+        case 0: *bm = T::V0::BM; *bn = T::V0::BN; break;
+        case 1: *bm = T::V1::BM; *bn = T::V1::BN; break;
+        case 2: *bm = T::V2::BM; *bn = T::V2::BN; break;
+        default: *bm = T::V3::BM; *bn = T::V3::BN; break;
+    }
+}
+}  // namespace
+
+int mfma_grouped_tile(int mode, int variant, int *bm, int *bn) {
+    if (variant < 0 || variant > 3) return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: tile_of<1>(variant, bm, bn); return UGEMM_OK;
+        case UGEMM_MODE_BF16X3: tile_of<2>(variant, bm, bn); return UGEMM_OK;
+        case UGEMM_MODE_BF16X6: tile_of<3>(variant, bm, bn); return UGEMM_OK;
+        default: return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided", mode);
+    }
+}
+
+int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
+                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
+    if (ntiles == 0) return UGEMM_OK;
+    switch (mode) {
+        case UGEMM_MODE_BF16X1: return grouped_np<1>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case UGEMM_MODE_BF16X3: return grouped_np<2>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case UGEMM_MODE_BF16X6: return grouped_np<3>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        default: return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided", mode);
+    }
 }
 
 int launch_mfma_conv(const Gemm<float> &g, const ConvDesc &cd, int mode, int variant, hipStream_t s) {

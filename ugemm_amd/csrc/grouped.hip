@@ -194,6 +194,15 @@ struct ugemm_grouped_plan {
     bool capture = false;
     bool conv = false;          // a plan of convolutions (grouped_fp32_kernel<CF, true>)
     int2 *d_ktab = nullptr;     // the convolutions' im2col offset tables, one block per problem
+    // Split-form plans (BF16 forms): every problem's planes, prepared by one
+    // job-table kernel, then one split-core block per tile.
+    bool split = false;
+    std::vector<char> ta, tb;   // layouts (the FP32 DevProblem keeps them in `layout`)
+    void *d_planes = nullptr;
+    PrepJob *d_jobs = nullptr;
+    GroupedSplitProblem *d_sprobs = nullptr;
+    GroupedSplitTile *d_stiles = nullptr;
+    int njobs = 0, prep_blocks = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -209,13 +218,24 @@ void release_schedule(ugemm_grouped_plan *p) {
     hipFree(p->d_tiles);
     hipFree(p->d_queues);
     hipFree(p->d_heads);
+    hipFree(p->d_planes);
+    hipFree(p->d_jobs);
+    hipFree(p->d_sprobs);
+    hipFree(p->d_stiles);
     p->d_tiles = nullptr;
     p->d_queues = nullptr;
     p->d_heads = nullptr;
+    p->d_planes = nullptr;
+    p->d_jobs = nullptr;
+    p->d_sprobs = nullptr;
+    p->d_stiles = nullptr;
 }
 
 int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (p->grid == 0) return UGEMM_OK;
+    if (p->split)
+        return launch_mfma_grouped(p->mode, p->variant, p->d_jobs, p->njobs, p->prep_blocks, p->d_sprobs,
+                                   p->d_stiles, p->ntiles, s);
     const VariantOps ops = variant_ops(p->variant);
     hipLaunchKernelGGL(grouped_reset_heads, dim3(1), dim3(64), 0, s, p->d_heads);
     hipError_t e = hipGetLastError();
@@ -227,6 +247,8 @@ int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (e != hipSuccess) return hip_fail(e, "grouped launch");
     return UGEMM_OK;
 }
+
+int capture_plan(ugemm_grouped_plan *p);   // records the plan's launches in its graph
 
 // Builds the tile table and the per-XCD tile queues for variant v,
 // uploads it, and (if requested) captures the launch in a graph.
@@ -285,20 +307,136 @@ int build_schedule(ugemm_grouped_plan *p, int v) {
     UGEMM_HIP_TRY(hipMalloc(&p->d_heads, kQueues * sizeof(int)));
     UGEMM_HIP_TRY(hipMemcpy(p->d_tiles, flat.data(), flat.size() * sizeof(DevTile), hipMemcpyHostToDevice));
     UGEMM_HIP_TRY(hipMemcpy(p->d_queues, qbegin.data(), qbegin.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (p->capture) {
-        UGEMM_HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_plan_kernel(p, p->cap_stream);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
-        if (rc) {
-            if (g) hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        p->graph = g;
-        UGEMM_HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-    }
+    if (p->capture) return capture_plan(p);
     return UGEMM_OK;
+}
+
+int capture_plan(ugemm_grouped_plan *p) {
+    UGEMM_HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_plan_kernel(p, p->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    p->graph = g;
+    UGEMM_HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return UGEMM_OK;
+}
+
+// Split-form plan for variant v (0-3 of the form's split-core table): planes
+// of every problem (K padded to 64; a problem with K == 0 or alpha == 0 gets
+// zero planes, so the core computes beta * C), the preparation jobs, and the
+// tiles in grouped order sorted by K, longest first (the hardware dispatches
+// blocks in order, which balances them as they finish).
+int build_split_schedule(ugemm_grouped_plan *p, int v) {
+    release_schedule(p);
+    p->variant = v;
+    int bm = 0, bn = 0;
+    int rc = mfma_grouped_tile(p->mode, v, &bm, &bn);
+    if (rc) return rc;
+    const int np = p->mode == UGEMM_MODE_BF16X1 ? 1 : (p->mode == UGEMM_MODE_BF16X3 ? 2 : 3);
+    std::vector<GroupedSplitProblem> sp;
+    std::vector<PrepJob> jobs;
+    struct HostTile {
+        GroupedSplitTile t;
+        long cost;
+    };
+    std::vector<HostTile> tiles;
+    std::vector<long> a_off, b_off;
+    long total = 0, blocks = 0;   // plane elements, preparation blocks
+    auto add_job = [&](const float *X, long ld, bool kcont, int rows, int K, int rows_pad, int k_pad, long off,
+                       long plane) {
+        PrepJob j{};
+        j.X = X;
+        j.ld = ld;
+        j.out = (void *)(intptr_t)off;   // element offset; rebased on the planes' address below
+        j.plane_stride = plane;
+        j.rows = rows;
+        j.K = K;
+        j.rows_pad = rows_pad;
+        j.k_pad = k_pad;
+        j.kcont = kcont ? 1 : 0;
+        j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
+        j.block0 = (int)blocks;
+        j.nbx = rows_pad / 64;
+        blocks += kcont ? ((long)rows_pad * (k_pad / 8) + 255) / 256 : (long)(rows_pad / 64) * (k_pad / 64);
+        jobs.push_back(j);
+    };
+    for (int i = 0; i < (int)p->probs.size(); ++i) {
+        const DevProblem &q = p->probs[i];
+        if (q.M == 0 || q.N == 0) continue;
+        if (q.K == 0 && q.beta == 1.0f) continue;   // nothing to do
+        const long Mp = (q.M + bm - 1) / bm * (long)bm, Np = (q.N + bn - 1) / bn * (long)bn;
+        const long Kp = std::max<long>(64, (q.K + 63) / 64 * 64);
+        const long a_plane = Mp * Kp, b_plane = Np * Kp;
+        const long ao = total, bo = total + np * a_plane;
+        total = bo + np * b_plane;
+        const bool ta = (q.layout & 1) != 0, tb = (q.layout & 2) != 0;
+        add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane);
+        add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane);
+        GroupedSplitProblem g{};
+        g.Ap = (const void *)(intptr_t)ao;
+        g.Bp = (const void *)(intptr_t)bo;
+        g.a_plane = a_plane;
+        g.b_plane = b_plane;
+        g.M = q.M;
+        g.N = q.N;
+        g.Kp = (int)Kp;
+        g.Mp = (int)Mp;
+        g.Np = (int)Np;
+        g.alpha = q.alpha;
+        g.beta = q.beta;
+        g.C = q.C;
+        g.ldc = q.ldc;
+        const int idx = (int)sp.size();
+        sp.push_back(g);
+        constexpr int GROUP_M = 8;
+        const int tiles_m = (int)(Mp / bm), tiles_n = (int)(Np / bn);
+        for (int g0 = 0; g0 < tiles_m; g0 += GROUP_M) {
+            const int g1 = std::min(tiles_m, g0 + GROUP_M);
+            for (int tn = 0; tn < tiles_n; ++tn)
+                for (int tm = g0; tm < g1; ++tm) tiles.push_back(HostTile{GroupedSplitTile{idx, tm * bm, tn * bn, 0}, Kp});
+        }
+    }
+    if (blocks > 0x7fffffffL || (long)tiles.size() > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "grouped: plan too large");
+    std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile &a, const HostTile &b) { return a.cost > b.cost; });
+    p->grid = tiles.empty() ? 0 : 1;
+    p->ntiles = (int)tiles.size();
+    p->njobs = (int)jobs.size();
+    p->prep_blocks = (int)blocks;
+    if (tiles.empty()) return UGEMM_OK;
+    // Planes (16-bit elements), then rebase the offsets on their address.
+    UGEMM_HIP_TRY(hipMalloc(&p->d_planes, (size_t)total * 2));
+    char *base = (char *)p->d_planes;
+    for (PrepJob &j : jobs) j.out = base + (intptr_t)j.out * 2;
+    for (GroupedSplitProblem &g : sp) {
+        g.Ap = base + (intptr_t)g.Ap * 2;
+        g.Bp = base + (intptr_t)g.Bp * 2;
+    }
+    std::vector<GroupedSplitTile> flat(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) flat[i] = tiles[i].t;
+    UGEMM_HIP_TRY(hipMalloc(&p->d_jobs, jobs.size() * sizeof(PrepJob)));
+    UGEMM_HIP_TRY(hipMalloc(&p->d_sprobs, sp.size() * sizeof(GroupedSplitProblem)));
+    UGEMM_HIP_TRY(hipMalloc(&p->d_stiles, flat.size() * sizeof(GroupedSplitTile)));
+    UGEMM_HIP_TRY(hipMemcpy(p->d_jobs, jobs.data(), jobs.size() * sizeof(PrepJob), hipMemcpyHostToDevice));
+    UGEMM_HIP_TRY(hipMemcpy(p->d_sprobs, sp.data(), sp.size() * sizeof(GroupedSplitProblem), hipMemcpyHostToDevice));
+    UGEMM_HIP_TRY(hipMemcpy(p->d_stiles, flat.data(), flat.size() * sizeof(GroupedSplitTile), hipMemcpyHostToDevice));
+    if (p->capture) return capture_plan(p);
+    return UGEMM_OK;
+}
+
+int default_split_variant(const ugemm_grouped_plan *p) {
+    // Most of the work in problems of 128+ rows and columns: 128 x 128 tiles; otherwise 64 x 64.
+    double big = 0, all = 0;
+    for (const DevProblem &q : p->probs) {
+        const double f = 2.0 * q.M * q.N * q.K;
+        all += f;
+        if (q.M >= 128 && q.N >= 128) big += f;
+    }
+    return (all == 0 || big >= 0.5 * all) ? 1 : 3;
 }
 
 bool parse_t(char c, bool *t) {
@@ -338,12 +476,16 @@ int finish_plan(ugemm_grouped_plan *p, int count, int capture, ugemm_grouped_pla
             return hip_fail(e, "grouped: capture stream");
         }
     }
-    // A forced single-GEMM variant applies when the grouped kernel has it too.
-    int v = forced_variant(UGEMM_MODE_FP32);
-    if (v >= kNumVariants) v = -1;
-    if (v < 0) v = tuned_variant(TUNE_KEY_GROUPED, p->conv, false, count, (int)std::min(p->flops / 1e6, 2e9), 0);
-    if (v < 0) v = default_variant(p);
-    const int rc = build_schedule(p, v);
+    // A forced single-GEMM variant applies when the grouped kernel has it too
+    // (split forms: the 4-wave split-core variants 0-3).
+    const int nv = p->split ? 4 : kNumVariants;
+    int v = forced_variant(p->mode);
+    if (v >= nv) v = -1;
+    if (v < 0)
+        v = tuned_variant(TUNE_KEY_GROUPED, p->conv, p->split ? p->mode : 0, count, (int)std::min(p->flops / 1e6, 2e9),
+                          0);
+    if (v < 0 || v >= nv) v = p->split ? default_split_variant(p) : default_variant(p);
+    const int rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
     if (rc) {
         ugemm_hip_grouped_plan_destroy(p);
         return rc;
@@ -361,12 +503,14 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
     if (!plan) return fail(UGEMM_ERR_ARG, "grouped: NULL plan pointer");
     *plan = nullptr;
     if (count < 0 || (count > 0 && !problems)) return fail(UGEMM_ERR_ARG, "grouped: bad problem list");
-    if (mode != UGEMM_MODE_FP32)
-        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32 only)", mode);
+    const bool split = mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6;
+    if (mode != UGEMM_MODE_FP32 && !split)
+        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32, BF16X1, BF16X3, BF16X6)", mode);
     int rc = ensure_device();
     if (rc) return rc;
     ugemm_grouped_plan *p = new ugemm_grouped_plan();
     p->mode = mode;
+    p->split = split;
     for (int i = 0; i < count; ++i) {
         const ugemm_problem_t &q = problems[i];
         bool ta, tb;
@@ -495,13 +639,14 @@ int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode
     return UGEMM_OK;
 }
 
-// Times each tile variant on the plan's own problems (C is overwritten, as by
-// a run) and keeps the fastest; the choice is remembered for plans with the
-// same problem count and work bucket.  Returns the chosen variant.
 int ugemm_hip_grouped_variant_count(void) { return kNumVariants; }
 
 const char *ugemm_hip_grouped_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kGroupedNames[v] : nullptr; }
 
+// Times each tile variant on the plan's own problems (C is overwritten, as by
+// a run) and keeps the fastest; the choice is remembered for plans with the
+// same problem count, work bucket and form.  Returns the chosen variant
+// (split-form plans: the split-core variant, 0-3).
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (!p) return fail(UGEMM_ERR_ARG, "grouped: NULL plan");
     if (p->grid == 0) return p->variant;
@@ -513,8 +658,9 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(UGEMM_ERR_HIP, "events");
     const bool cap = p->capture;
     p->capture = false;
-    for (int v = 0; v < kNumVariants && !rc; ++v) {
-        rc = build_schedule(p, v);
+    const int nv = p->split ? 4 : kNumVariants;
+    for (int v = 0; v < nv && !rc; ++v) {
+        rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
         if (rc) break;
         launch_plan_kernel(p, s);   // warm-up
         hipEventRecord(e0, s);
@@ -532,14 +678,14 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
         }
     }
     p->capture = cap;
-    if (!rc) rc = build_schedule(p, best);
+    if (!rc) rc = p->split ? build_split_schedule(p, best) : build_schedule(p, best);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     hipStreamSynchronize(s);
     hipStreamDestroy(s);
     if (rc) return rc;
-    tune_record(TUNE_KEY_GROUPED, p->conv, false, (int)p->probs.size(), (int)std::min(p->flops / 1e6, 2e9), 0, best,
-                best_ms / 3);
+    tune_record(TUNE_KEY_GROUPED, p->conv, p->split ? p->mode : 0, (int)p->probs.size(),
+                (int)std::min(p->flops / 1e6, 2e9), 0, best, best_ms / 3);
     return best;
 }
 

@@ -35,6 +35,33 @@ struct ConvDesc {
     int H, W, wcol, stride, pad;
 };
 
+// A problem of a grouped split-form plan: its prepared planes (blocked
+// layout, NP planes of Mp x Kp and Np x Kp) and C; and one C tile of it.
+struct GroupedSplitProblem {
+    const void *Ap, *Bp;
+    long a_plane, b_plane;
+    int M, N, Kp, Mp, Np;
+    float alpha, beta;
+    float *C;
+    long ldc;
+};
+struct GroupedSplitTile {
+    int problem, m0, n0, pad;
+};
+// One operand of a grouped split-form plan to prepare: its FP32 source and
+// its planes, and the first block of the job in the preparation grid.
+// k-contiguous jobs take one 8-k chunk per thread (256 per block);
+// row-contiguous jobs a 64 x 64 tile per block (nbx tiles along the rows).
+struct PrepJob {
+    const float *X;
+    long ld;
+    void *out;
+    long plane_stride;
+    int rows, K, rows_pad, k_pad;
+    int kcont, vec;
+    int block0, nbx;
+};
+
 // One row-major GEMM after major/transpose normalisation:
 //   C[m, n] = alpha * sum_k opA(m, k) * opB(k, n) + beta * C[m, n]
 //   opA(m, k) = ta ? A[k * lda + m] : A[m * lda + k]
@@ -74,6 +101,11 @@ int mfma_variant_count(int mode);
 const char *mfma_variant_name(int mode, int v);
 int mfma_pick_variant(const Gemm<float> &g, int mode);
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
+// Grouped split-form plans: tile shape of a variant (0-3), and the launch of
+// the preparation (jobs) then the core (one block per tile).
+int mfma_grouped_tile(int mode, int variant, int *bm, int *bn);
+int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
+                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s);
 // Convolution in a split form: g.A = weights (g.M x g.K, k-contiguous), g.B
 // unused -- the im2col operand of cd is gathered into planes.
 int launch_mfma_conv(const Gemm<float> &g, const ConvDesc &cd, int mode, int variant, hipStream_t s);
@@ -127,10 +159,11 @@ int grouped_variant_count();
 // ---- tuning (tune.hip) ----
 // Tuning-table key of grouped plans (variants = grouped_variant_count()).
 constexpr int TUNE_KEY_GROUPED = 16;
-int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits = nullptr);   // -1 if none
+// ta, tb: layout flags (grouped keys: convolution flag, split form).
+int tuned_variant(int mode, int ta, int tb, int M, int N, int K, int *splits = nullptr);   // -1 if none
 int forced_variant(int mode);                                         // -1 if none
 int forced_splits(int mode);                                          // 0 if none
-void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms, int splits = 0);
+void tune_record(int mode, int ta, int tb, int M, int N, int K, int variant, float ms, int splits = 0);
 
 // ---- launch timing (runtime.hip) ----
 // When enabled (ugemm_hip_timing_enable), the dominant kernel of every call

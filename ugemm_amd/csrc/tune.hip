@@ -43,7 +43,7 @@ int lg(int x) {
     while ((1L << b) < (long)x) ++b;
     return b;
 }
-Key key(int mode, bool ta, bool tb, int M, int N, int K) { return Key(mode, ta, tb, lg(M), lg(N), lg(K)); }
+Key key(int mode, int ta, int tb, int M, int N, int K) { return Key(mode, ta, tb, lg(M), lg(N), lg(K)); }
 int slot(int mode) { return mode < 0 ? UGEMM_MODE_COUNT : mode; }
 
 __global__ void fill_uniform(float *p, long n, unsigned seed) {
@@ -71,7 +71,7 @@ void arch_info(char *arch, size_t n, int *cus) {
 
 }  // namespace
 
-int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits) {
+int tuned_variant(int mode, int ta, int tb, int M, int N, int K, int *splits) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_table.find(key(mode, ta, tb, M, N, K));
     if (splits) *splits = it == g_table.end() ? 0 : it->second.splits;
@@ -81,7 +81,7 @@ int tuned_variant(int mode, bool ta, bool tb, int M, int N, int K, int *splits) 
 int forced_variant(int mode) { return t_forced_p1[slot(mode)] - 1; }
 int forced_splits(int mode) { return t_forced_splits[slot(mode)]; }
 
-void tune_record(int mode, bool ta, bool tb, int M, int N, int K, int variant, float ms, int splits) {
+void tune_record(int mode, int ta, int tb, int M, int N, int K, int variant, float ms, int splits) {
     std::lock_guard<std::mutex> lk(g_mu);
     g_table[key(mode, ta, tb, M, N, K)] = Entry{variant, ms, splits};
 }
