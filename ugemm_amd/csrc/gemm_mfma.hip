@@ -1153,9 +1153,9 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
 }
 
 // ---------------------------------------------------------------------------
-// Grouped split-form plans (grouped.hip): the 4-wave variants V0-V3, whose
-// tiles suit uneven batches; BF16 forms only (the FP16 form's row scales
-// are not prepared by the grouped preparation kernel).
+// Grouped split-form plans (grouped.hip): any variant (no operand is split
+// in the core here); BF16 forms only (the FP16 form's row scales are not
+// prepared by the grouped preparation kernel).
 namespace {
 template <class CF>
 int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
@@ -1187,6 +1187,9 @@ int grouped_np(int variant, const PrepJob *jobs, int njobs, int prep_blocks, con
         case 1: return grouped_core<typename T::V1>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
         case 2: return grouped_core<typename T::V2>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
         case 3: return grouped_core<typename T::V3>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 4: return grouped_core<typename T::V4>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 5: return grouped_core<typename T::V5>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 6: return grouped_core<typename T::V6>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
         default: return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     }
 }
@@ -1198,13 +1201,16 @@ void tile_of(int variant, int *bm, int *bn) {
         case 0: *bm = T::V0::BM; *bn = T::V0::BN; break;
         case 1: *bm = T::V1::BM; *bn = T::V1::BN; break;
         case 2: *bm = T::V2::BM; *bn = T::V2::BN; break;
-        default: *bm = T::V3::BM; *bn = T::V3::BN; break;
+        case 3: *bm = T::V3::BM; *bn = T::V3::BN; break;
+        case 4: *bm = T::V4::BM; *bn = T::V4::BN; break;
+        case 5: *bm = T::V5::BM; *bn = T::V5::BN; break;
+        default: *bm = T::V6::BM; *bn = T::V6::BN; break;
     }
 }
 }  // namespace
 
 int mfma_grouped_tile(int mode, int variant, int *bm, int *bn) {
-    if (variant < 0 || variant > 3) return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
+    if (variant < 0 || variant >= kNumVariants) return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     switch (mode) {
         case UGEMM_MODE_BF16X1: tile_of<1>(variant, bm, bn); return UGEMM_OK;
         case UGEMM_MODE_BF16X3: tile_of<2>(variant, bm, bn); return UGEMM_OK;

@@ -326,7 +326,7 @@ int capture_plan(ugemm_grouped_plan *p) {
     return UGEMM_OK;
 }
 
-// Split-form plan for variant v (0-3 of the form's split-core table): planes
+// Split-form plan for variant v of the form's split-core table: planes
 // of every problem (K padded to 64; a problem with K == 0 or alpha == 0 gets
 // zero planes, so the core computes beta * C), the preparation jobs, and the
 // tiles in grouped order sorted by K, longest first (the hardware dispatches
@@ -477,8 +477,8 @@ int finish_plan(ugemm_grouped_plan *p, int count, int capture, ugemm_grouped_pla
         }
     }
     // A forced single-GEMM variant applies when the grouped kernel has it too
-    // (split forms: the 4-wave split-core variants 0-3).
-    const int nv = p->split ? 4 : kNumVariants;
+    // (split forms: every split-core variant).
+    const int nv = p->split ? mfma_variant_count(p->mode) : kNumVariants;
     int v = forced_variant(p->mode);
     if (v >= nv) v = -1;
     if (v < 0)
@@ -646,7 +646,7 @@ const char *ugemm_hip_grouped_variant_name(int v) { return (v >= 0 && v < kNumVa
 // Times each tile variant on the plan's own problems (C is overwritten, as by
 // a run) and keeps the fastest; the choice is remembered for plans with the
 // same problem count, work bucket and form.  Returns the chosen variant
-// (split-form plans: the split-core variant, 0-3).
+// (split-form plans: the split-core variant).
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (!p) return fail(UGEMM_ERR_ARG, "grouped: NULL plan");
     if (p->grid == 0) return p->variant;
@@ -658,7 +658,7 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(UGEMM_ERR_HIP, "events");
     const bool cap = p->capture;
     p->capture = false;
-    const int nv = p->split ? 4 : kNumVariants;
+    const int nv = p->split ? mfma_variant_count(p->mode) : kNumVariants;
     for (int v = 0; v < nv && !rc; ++v) {
         rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
         if (rc) break;

@@ -101,7 +101,7 @@ int mfma_variant_count(int mode);
 const char *mfma_variant_name(int mode, int v);
 int mfma_pick_variant(const Gemm<float> &g, int mode);
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
-// Grouped split-form plans: tile shape of a variant (0-3), and the launch of
+// Grouped split-form plans: tile shape of a variant, and the launch of
 // the preparation (jobs) then the core (one block per tile).
 int mfma_grouped_tile(int mode, int variant, int *bm, int *bn);
 int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
