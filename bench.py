@@ -253,29 +253,35 @@ def config4(stream, fp32_single_tf):
     e1.synchronize()
     ms_loop = e0.elapsed_ms(e1) / loop_reps
     plan.destroy()
-    # The same batch in the BF16x3 form: one preparation launch for every
-    # operand, one split-core block per tile (captured, tuned); its own
-    # bound (2^-14 per product) on the same problems.
-    plan3 = ug.GroupedPlan(probs, mode=ug.MODE_BF16X3, capture=True)
-    tuned3 = plan3.tune()
-    for _ in range(2):
-        plan3.run(stream)
-    stream.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        plan3.run(stream)
-    e1.record(stream)
-    e1.synchronize()
-    ms_group3 = e0.elapsed_ms(e1) / reps
-    worst3 = 0.0
-    for i in np.argsort(Ms * Ns * Ks)[:6]:
-        M, N, K = int(Ms[i]), int(Ns[i]), int(Ks[i])
-        A = pa[a_off[i]:a_off[i] + M * K].reshape(M, K).astype(np.float64)
-        B = pb[b_off[i]:b_off[i] + K * N].reshape(K, N).astype(np.float64)
-        got = dC.download()[c_off[i]:c_off[i] + M * N].reshape(M, N).astype(np.float64)
-        bound = (2 * (K + 1) * 2.0 ** -24 + 2.0 ** -14) * (np.abs(A) @ np.abs(B))
-        worst3 = max(worst3, float(np.max(np.abs(got - A @ B) / bound)))
-    plan3.destroy()
+    # The same batch in the split forms: one preparation launch for every
+    # operand (FP16X3: after a row-maximum launch), one split-core block per
+    # tile (captured, tuned); each form's own bound on the same problems.
+    split_legs = {}
+    for form, extra in (("bf16x3", 2.0 ** -14), ("fp16x3", 2.0 ** -19)):
+        plan_s = ug.GroupedPlan(probs, mode=ug.MODES[form], capture=True)
+        tuned_s = plan_s.tune()
+        for _ in range(2):
+            plan_s.run(stream)
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            plan_s.run(stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms_s = e0.elapsed_ms(e1) / reps
+        worst_s = 0.0
+        for i in np.argsort(Ms * Ns * Ks)[:6]:
+            M, N, K = int(Ms[i]), int(Ns[i]), int(Ks[i])
+            A = pa[a_off[i]:a_off[i] + M * K].reshape(M, K).astype(np.float64)
+            B = pb[b_off[i]:b_off[i] + K * N].reshape(K, N).astype(np.float64)
+            got = dC.download()[c_off[i]:c_off[i] + M * N].reshape(M, N).astype(np.float64)
+            bound = (2 * (K + 1) * 2.0 ** -24 + extra) * (np.abs(A) @ np.abs(B))
+            worst_s = max(worst_s, float(np.max(np.abs(got - A @ B) / bound)))
+        plan_s.destroy()
+        split_legs[form + "_grouped"] = {"split_core_variant": ug.variants(ug.MODES[form])[tuned_s],
+                                         "grouped_graph_ms": round(ms_s, 4),
+                                         "grouped_tflops": round(flop / (ms_s * 1e-3) / 1e12, 3),
+                                         "checked_problems_worst_err_over_bound": worst_s}
     tf_group = flop / (ms_group * 1e-3) / 1e12
     r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
          "grouped_graph_ms": round(ms_group, 4), "grouped_tflops": round(tf_group, 3),
@@ -283,14 +289,11 @@ def config4(stream, fp32_single_tf):
          "loop_plain_ms": round(ms_loop, 4), "loop_plain_tflops": round(flop / (ms_loop * 1e-3) / 1e12, 3),
          "speedup_vs_loop": round(ms_loop / ms_group, 3),
          "frac_of_single_large_gemm_rate": round(tf_group / fp32_single_tf, 4) if fp32_single_tf else None,
-         "checked_problems_worst_err_over_bound": worst,
-         "bf16x3_grouped": {"split_core_variant": ug.variants(ug.MODE_BF16X3)[tuned3],
-                            "grouped_graph_ms": round(ms_group3, 4),
-                            "grouped_tflops": round(flop / (ms_group3 * 1e-3) / 1e12, 3),
-                            "checked_problems_worst_err_over_bound": worst3}}
+         "checked_problems_worst_err_over_bound": worst}
+    r.update(split_legs)
     log(f"[config4] grouped {tf_group:.1f} TFLOP/s ({ms_group:.3f} ms), loop {r['loop_plain_tflops']:.1f} TFLOP/s, "
         f"{r['frac_of_single_large_gemm_rate']} of single-GEMM rate; bf16x3 grouped "
-        f"{r['bf16x3_grouped']['grouped_tflops']:.1f} TFLOP/s")
+        f"{r['bf16x3_grouped']['grouped_tflops']:.1f}, fp16x3 grouped {r['fp16x3_grouped']['grouped_tflops']:.1f} TFLOP/s")
     for d in (dpa, dpb, dC):
         d.free()
     e0.destroy()

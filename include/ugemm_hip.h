@@ -146,9 +146,10 @@ typedef struct ugemm_grouped_plan ugemm_grouped_plan_t;
 /* Build a plan: validates the problems, builds the tile table and the
  * per-XCD tile queues, uploads them, and (capture != 0) records the launch
  * in a hipGraph so that ugemm_hip_grouped_run replays one graph launch.
- * mode: UGEMM_MODE_FP32, or a BF16 split form (UGEMM_MODE_BF16X1/X3/X6:
- * the plan owns every problem's prepared planes; a run is a preparation
- * launch and a core launch); FP16X3 and FP8 return UGEMM_ERR_UNSUPPORTED. */
+ * mode: UGEMM_MODE_FP32, or a split form (UGEMM_MODE_BF16X1/X3/X6,
+ * UGEMM_MODE_FP16X3: the plan owns every problem's prepared planes; a run is
+ * a preparation launch and a core launch, FP16X3 adding a row-maximum pass);
+ * FP8 returns UGEMM_ERR_UNSUPPORTED. */
 int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
                                   const ugemm_problem_t *problems, int mode,
                                   int capture, void *stream);

@@ -140,16 +140,17 @@ def test_grouped_mixed_tile_shapes():
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-SPLIT_GROUPED = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6]
+SPLIT_GROUPED = [ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3]
 
 
 @pytest.mark.parametrize("mode", SPLIT_GROUPED)
 @pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6])
 def test_grouped_split_forms(mode, variant):
-    """Grouped plans in the BF16 split forms: every problem's planes from one
-    preparation launch, one split-core block per tile; uneven im2col-shaped
-    problems of every layout (and the tiny / K = 0 / alpha = 0 edge
-    problems) meet the form's bound (helpers.MODE_EXTRA); -1 = default tile."""
+    """Grouped plans in the split forms: every problem's planes from one
+    preparation launch (FP16X3: after one row-maximum launch), one split-core
+    block per tile; uneven im2col-shaped problems of every layout (and the
+    tiny / K = 0 / alpha = 0 edge problems) meet the form's bound
+    (helpers.MODE_EXTRA); -1 = default tile."""
     cases = [DevCase(c) for c in uneven_cases(14, 50 + variant + 7 * mode)]
     ug.force_variant(mode, variant)
     try:
@@ -212,10 +213,10 @@ def test_grouped_argument_errors():
     assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP32, None) == ug.ERR_ARG
     ok = ug.Problem(b"N", b"N", 4, 4, 4, 1.0, 0.0, 16, 4, 16, 4, 16, 4)
     arr = (ug.Problem * 1)(ok)
-    # Forms without a grouped path are refused before any work (the pointers
-    # here are not device memory).
-    assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP16X3, None) == ug.ERR_UNSUPPORTED
+    # The form without a grouped path is refused before any work (the
+    # pointers here are not device memory).
     assert L.ugemm_hip_sgemm_grouped(1, arr, ug.MODE_FP8_BLOCK, None) == ug.ERR_UNSUPPORTED
+    assert L.ugemm_hip_sgemm_grouped(1, arr, 42, None) == ug.ERR_UNSUPPORTED
     assert L.ugemm_hip_sgemm_grouped(0, None, ug.MODE_FP32, None) == ug.OK
 
 

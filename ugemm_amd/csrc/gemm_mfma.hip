@@ -247,10 +247,10 @@ __global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restr
         const long f = (long)lb * 256 + threadIdx.x;
         if (f < (long)j.rows_pad * (j.k_pad / 8))
             prep_kcont_chunk<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
-                                      nullptr, f);
+                                      j.amax, f);
     } else {
         prep_rcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
-                                 nullptr, (lb % j.nbx) * 64, (lb / j.nbx) * 64, tile);
+                                 j.amax, (lb % j.nbx) * 64, (lb / j.nbx) * 64, tile);
     }
 }
 
@@ -262,9 +262,8 @@ __global__ void fill_zero_u32(unsigned *__restrict__ p, long n) {
 }
 
 // k-contiguous source: block b covers row b / kchunks, k in [2048 * (b % kchunks), +2048).
-__global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ X, long ld, int rows, int K,
-                                                      bool vec, long kchunks, unsigned *__restrict__ amax) {
-    const long b = blockIdx.x;
+__device__ __forceinline__ void row_amax_kcont_block(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                     long kchunks, unsigned *__restrict__ amax, long b) {
     const int r = (int)(b / kchunks);
     const int k = (int)(b % kchunks) * 2048 + threadIdx.x * 8;
     const float *p = X + (long)r * ld + k;
@@ -290,11 +289,15 @@ __global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                      bool vec, long kchunks, unsigned *__restrict__ amax) {
+    row_amax_kcont_block(X, ld, rows, K, vec, kchunks, amax, blockIdx.x);
+}
+
 // Row-contiguous source (X[k*ld + r]): block b covers columns r in
 // [256 * (b / kblocks), +256) and k in [256 * (b % kblocks), +256).
-__global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ X, long ld, int rows, int K,
-                                                      bool vec, long kblocks, unsigned *__restrict__ amax) {
-    const long b = blockIdx.x;
+__device__ __forceinline__ void row_amax_rcont_block(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                     long kblocks, unsigned *__restrict__ amax, long b) {
     const int c0 = (int)(b / kblocks) * 256, k0 = (int)(b % kblocks) * 256;
     const int cg = threadIdx.x & 63, kp = threadIdx.x >> 6;
     const int c = c0 + cg * 4;
@@ -318,6 +321,28 @@ __global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ 
     const int t = threadIdx.x, col = c0 + t;
     const float v = fmaxf(fmaxf(sm[0][t], sm[1][t]), fmaxf(sm[2][t], sm[3][t]));
     if (col < rows && v > 0.0f) atomicMax(&amax[col], __float_as_uint(v));
+}
+
+__global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ X, long ld, int rows, int K,
+                                                      bool vec, long kblocks, unsigned *__restrict__ amax) {
+    row_amax_rcont_block(X, ld, rows, K, vec, kblocks, amax, blockIdx.x);
+}
+
+// Row maxima of the operands of a grouped FP16X3 plan: job j (PrepJob with
+// out = its maxima, nbx = K chunks per row (k-contiguous: 2048 wide) or per
+// 256-column block (row-contiguous: 256 deep)), blocks as in the kernels above.
+__global__ __launch_bounds__(256) void grouped_row_amax(const PrepJob *__restrict__ jobs, int njobs) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].block0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PrepJob j = jobs[lo];
+    const long lb = (long)blockIdx.x - j.block0;
+    unsigned *amax = static_cast<unsigned *>(j.out);
+    if (j.kcont) row_amax_kcont_block(j.X, j.ld, j.rows, j.K, j.vec != 0, j.nbx, amax, lb);
+    else row_amax_rcont_block(j.X, j.ld, j.rows, j.K, j.vec != 0, j.nbx, amax, lb);
 }
 
 // Implicit im2col operand of a convolution (ConvDesc): element (n, k) of the
@@ -484,8 +509,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                                                        long a_plane_, int Mp_, const typename CF::FMT::E *__restrict__ Bp_,
                                                        long b_plane_, int Np_, float alpha_, float beta_,
                                                        float *__restrict__ C_, long ldc_, int tiles_m, int tiles_n,
-                                                       const unsigned *__restrict__ amax_a,
-                                                       const unsigned *__restrict__ amax_b, Epilogue epi,
+                                                       const unsigned *__restrict__ amax_a_,
+                                                       const unsigned *__restrict__ amax_b_, Epilogue epi,
                                                        const float *__restrict__ Af, long lda,
                                                        const float *__restrict__ Bf, long ldb, int K, int vec_ab,
                                                        int kt_per, float *__restrict__ ws,
@@ -494,7 +519,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
-    static_assert(!GR || (FA == 0 && FB == 0 && !FMT::SCALED), "grouped plans stream prepared, unscaled planes");
+    static_assert(!GR || (FA == 0 && FB == 0), "grouped plans stream prepared planes");
     const GroupedSplitTile gt = GR ? gtiles[blockIdx.x] : GroupedSplitTile{0, 0, 0, 0};
     const GroupedSplitProblem *gp = GR ? gprobs + gt.problem : nullptr;
     const int M = GR ? gp->M : M_, N = GR ? gp->N : N_, Kp = GR ? gp->Kp : Kp_;
@@ -505,6 +530,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     const float alpha = GR ? gp->alpha : alpha_, beta = GR ? gp->beta : beta_;
     float *__restrict__ C = GR ? gp->C : C_;
     const long ldc = GR ? gp->ldc : ldc_;
+    const unsigned *__restrict__ amax_a = GR ? gp->amax_a : amax_a_;
+    const unsigned *__restrict__ amax_b = GR ? gp->amax_b : amax_b_;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
     constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -1154,13 +1181,19 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
 
 // ---------------------------------------------------------------------------
 // Grouped split-form plans (grouped.hip): any variant (no operand is split
-// in the core here); BF16 forms only (the FP16 form's row scales are not
-// prepared by the grouped preparation kernel).
+// in the core here); the FP16 form adds the row-maximum pass.
 namespace {
 template <class CF>
 int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
-                 const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
+                 const GroupedSplitTile *tiles, int ntiles, hipStream_t s, const GroupedScales *sc) {
     using FMT = typename CF::FMT;
+    if (FMT::SCALED) {
+        // Row maxima first (a fill kernel, not a memset: the launches are captured in graphs).
+        hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((sc->count + 255) / 256, 4096)), dim3(256), 0,
+                           s, sc->amax, sc->count);
+        if (sc->blocks > 0)
+            hipLaunchKernelGGL(grouped_row_amax, dim3((unsigned)sc->blocks), dim3(256), 0, s, sc->jobs, sc->njobs);
+    }
     hipLaunchKernelGGL((grouped_split_prep<CF::NP, FMT>), dim3((unsigned)prep_blocks), dim3(256), 0, s, jobs, njobs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "grouped split-form preparation");
@@ -1178,25 +1211,25 @@ int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedS
     return e == hipSuccess ? UGEMM_OK : hip_fail(e, "grouped split-form core");
 }
 
-template <int NP>
+template <int NP, class FMT = FmtBf16>
 int grouped_np(int variant, const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
-               const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
-    using T = Table<NP>;
+               const GroupedSplitTile *tiles, int ntiles, hipStream_t s, const GroupedScales *sc) {
+    using T = Table<NP, FMT>;
     switch (variant) {
-        case 0: return grouped_core<typename T::V0>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 1: return grouped_core<typename T::V1>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 2: return grouped_core<typename T::V2>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 3: return grouped_core<typename T::V3>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 4: return grouped_core<typename T::V4>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 5: return grouped_core<typename T::V5>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case 6: return grouped_core<typename T::V6>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case 0: return grouped_core<typename T::V0>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 1: return grouped_core<typename T::V1>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 2: return grouped_core<typename T::V2>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 3: return grouped_core<typename T::V3>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 4: return grouped_core<typename T::V4>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 5: return grouped_core<typename T::V5>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 6: return grouped_core<typename T::V6>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
         default: return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     }
 }
 
-template <int NP>
+template <int NP, class FMT = FmtBf16>
 void tile_of(int variant, int *bm, int *bn) {
-    using T = Table<NP>;
+    using T = Table<NP, FMT>;
     switch (variant) {
         case 0: *bm = T::V0::BM; *bn = T::V0::BN; break;
         case 1: *bm = T::V1::BM; *bn = T::V1::BN; break;
@@ -1215,17 +1248,22 @@ int mfma_grouped_tile(int mode, int variant, int *bm, int *bn) {
         case UGEMM_MODE_BF16X1: tile_of<1>(variant, bm, bn); return UGEMM_OK;
         case UGEMM_MODE_BF16X3: tile_of<2>(variant, bm, bn); return UGEMM_OK;
         case UGEMM_MODE_BF16X6: tile_of<3>(variant, bm, bn); return UGEMM_OK;
+        case UGEMM_MODE_FP16X3: tile_of<2, FmtF16>(variant, bm, bn); return UGEMM_OK;
         default: return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided", mode);
     }
 }
 
 int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
-                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s) {
+                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s,
+                        const GroupedScales *sc) {
     if (ntiles == 0) return UGEMM_OK;
     switch (mode) {
-        case UGEMM_MODE_BF16X1: return grouped_np<1>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case UGEMM_MODE_BF16X3: return grouped_np<2>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
-        case UGEMM_MODE_BF16X6: return grouped_np<3>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s);
+        case UGEMM_MODE_BF16X1: return grouped_np<1>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_BF16X3: return grouped_np<2>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_BF16X6: return grouped_np<3>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_FP16X3:
+            if (!sc) return fail(UGEMM_ERR_ARG, "grouped FP16X3: no row-scale jobs");
+            return grouped_np<2, FmtF16>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
         default: return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided", mode);
     }
 }

@@ -203,6 +203,11 @@ struct ugemm_grouped_plan {
     GroupedSplitProblem *d_sprobs = nullptr;
     GroupedSplitTile *d_stiles = nullptr;
     int njobs = 0, prep_blocks = 0;
+    // FP16X3: every operand row's maximum (the row scales), and its jobs.
+    unsigned *d_amax = nullptr;
+    PrepJob *d_ajobs = nullptr;
+    long amax_count = 0, amax_blocks = 0;
+    int najobs = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -222,6 +227,10 @@ void release_schedule(ugemm_grouped_plan *p) {
     hipFree(p->d_jobs);
     hipFree(p->d_sprobs);
     hipFree(p->d_stiles);
+    hipFree(p->d_amax);
+    hipFree(p->d_ajobs);
+    p->d_amax = nullptr;
+    p->d_ajobs = nullptr;
     p->d_tiles = nullptr;
     p->d_queues = nullptr;
     p->d_heads = nullptr;
@@ -233,9 +242,11 @@ void release_schedule(ugemm_grouped_plan *p) {
 
 int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (p->grid == 0) return UGEMM_OK;
-    if (p->split)
+    if (p->split) {
+        const GroupedScales sc{p->d_amax, p->amax_count, p->d_ajobs, p->najobs, p->amax_blocks};
         return launch_mfma_grouped(p->mode, p->variant, p->d_jobs, p->njobs, p->prep_blocks, p->d_sprobs,
-                                   p->d_stiles, p->ntiles, s);
+                                   p->d_stiles, p->ntiles, s, p->d_amax ? &sc : nullptr);
+    }
     const VariantOps ops = variant_ops(p->variant);
     hipLaunchKernelGGL(grouped_reset_heads, dim3(1), dim3(64), 0, s, p->d_heads);
     hipError_t e = hipGetLastError();
@@ -337,9 +348,28 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     int bm = 0, bn = 0;
     int rc = mfma_grouped_tile(p->mode, v, &bm, &bn);
     if (rc) return rc;
-    const int np = p->mode == UGEMM_MODE_BF16X1 ? 1 : (p->mode == UGEMM_MODE_BF16X3 ? 2 : 3);
+    const int np = p->mode == UGEMM_MODE_BF16X1 ? 1 : (p->mode == UGEMM_MODE_BF16X6 ? 3 : 2);
+    const bool scaled = p->mode == UGEMM_MODE_FP16X3;
     std::vector<GroupedSplitProblem> sp;
-    std::vector<PrepJob> jobs;
+    std::vector<PrepJob> jobs, ajobs;
+    long acount = 0, ablocks = 0;   // row maxima, row-maximum blocks (FP16X3)
+    // Row-maximum job of an operand (FP16X3): its maxima at element offset
+    // `at` of the maxima buffer (rebased below).
+    auto add_amax_job = [&](const float *X, long ld, bool kcont, int rows, int K, long at) {
+        PrepJob j{};
+        j.X = X;
+        j.ld = ld;
+        j.out = (void *)(intptr_t)at;
+        j.rows = rows;
+        j.K = K;
+        j.kcont = kcont ? 1 : 0;
+        j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
+        j.block0 = (int)ablocks;
+        j.nbx = kcont ? (K + 2047) / 2048 : (K + 255) / 256;
+        ablocks += kcont ? (long)rows * j.nbx : (long)((rows + 255) / 256) * j.nbx;
+        if (K > 0) ajobs.push_back(j);
+        else ablocks = j.block0;   // no rows to scan
+    };
     struct HostTile {
         GroupedSplitTile t;
         long cost;
@@ -348,8 +378,9 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     std::vector<long> a_off, b_off;
     long total = 0, blocks = 0;   // plane elements, preparation blocks
     auto add_job = [&](const float *X, long ld, bool kcont, int rows, int K, int rows_pad, int k_pad, long off,
-                       long plane) {
+                       long plane, long amax_at) {
         PrepJob j{};
+        j.amax = scaled ? (const unsigned *)(intptr_t)amax_at : nullptr;
         j.X = X;
         j.ld = ld;
         j.out = (void *)(intptr_t)off;   // element offset; rebased on the planes' address below
@@ -375,9 +406,17 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         const long ao = total, bo = total + np * a_plane;
         total = bo + np * b_plane;
         const bool ta = (q.layout & 1) != 0, tb = (q.layout & 2) != 0;
-        add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane);
-        add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane);
+        const long am_a = acount, am_b = acount + Mp;
+        if (scaled) {
+            acount += Mp + Np;
+            add_amax_job(q.A, q.lda, !ta, q.M, q.K, am_a);
+            add_amax_job(q.B, q.ldb, tb, q.N, q.K, am_b);
+        }
+        add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane, am_a);
+        add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane, am_b);
         GroupedSplitProblem g{};
+        g.amax_a = scaled ? (const unsigned *)(intptr_t)am_a : nullptr;
+        g.amax_b = scaled ? (const unsigned *)(intptr_t)am_b : nullptr;
         g.Ap = (const void *)(intptr_t)ao;
         g.Bp = (const void *)(intptr_t)bo;
         g.a_plane = a_plane;
@@ -401,7 +440,8 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
                 for (int tm = g0; tm < g1; ++tm) tiles.push_back(HostTile{GroupedSplitTile{idx, tm * bm, tn * bn, 0}, Kp});
         }
     }
-    if (blocks > 0x7fffffffL || (long)tiles.size() > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "grouped: plan too large");
+    if (blocks > 0x7fffffffL || ablocks > 0x7fffffffL || (long)tiles.size() > 0x7fffffffL)
+        return fail(UGEMM_ERR_ARG, "grouped: plan too large");
     std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile &a, const HostTile &b) { return a.cost > b.cost; });
     p->grid = tiles.empty() ? 0 : 1;
     p->ntiles = (int)tiles.size();
@@ -415,6 +455,24 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     for (GroupedSplitProblem &g : sp) {
         g.Ap = base + (intptr_t)g.Ap * 2;
         g.Bp = base + (intptr_t)g.Bp * 2;
+    }
+    if (scaled) {
+        // Row maxima: rebase the offsets on the buffer's address.
+        UGEMM_HIP_TRY(hipMalloc(&p->d_amax, (size_t)std::max<long>(1, acount) * sizeof(unsigned)));
+        unsigned *am = p->d_amax;
+        for (PrepJob &j : jobs) j.amax = am + (intptr_t)j.amax;
+        for (PrepJob &j : ajobs) j.out = am + (intptr_t)j.out;
+        for (GroupedSplitProblem &g : sp) {
+            g.amax_a = am + (intptr_t)g.amax_a;
+            g.amax_b = am + (intptr_t)g.amax_b;
+        }
+        p->amax_count = acount;
+        p->amax_blocks = ablocks;
+        p->najobs = (int)ajobs.size();
+        if (!ajobs.empty()) {
+            UGEMM_HIP_TRY(hipMalloc(&p->d_ajobs, ajobs.size() * sizeof(PrepJob)));
+            UGEMM_HIP_TRY(hipMemcpy(p->d_ajobs, ajobs.data(), ajobs.size() * sizeof(PrepJob), hipMemcpyHostToDevice));
+        }
     }
     std::vector<GroupedSplitTile> flat(tiles.size());
     for (size_t i = 0; i < tiles.size(); ++i) flat[i] = tiles[i].t;
@@ -503,9 +561,10 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
     if (!plan) return fail(UGEMM_ERR_ARG, "grouped: NULL plan pointer");
     *plan = nullptr;
     if (count < 0 || (count > 0 && !problems)) return fail(UGEMM_ERR_ARG, "grouped: bad problem list");
-    const bool split = mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6;
+    const bool split = mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6 ||
+                       mode == UGEMM_MODE_FP16X3;
     if (mode != UGEMM_MODE_FP32 && !split)
-        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32, BF16X1, BF16X3, BF16X6)", mode);
+        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32 and the split forms)", mode);
     int rc = ensure_device();
     if (rc) return rc;
     ugemm_grouped_plan *p = new ugemm_grouped_plan();

@@ -44,6 +44,7 @@ struct GroupedSplitProblem {
     float alpha, beta;
     float *C;
     long ldc;
+    const unsigned *amax_a, *amax_b;   // FP16X3: per-row maxima (Mp, Np entries)
 };
 struct GroupedSplitTile {
     int problem, m0, n0, pad;
@@ -60,6 +61,16 @@ struct PrepJob {
     int rows, K, rows_pad, k_pad;
     int kcont, vec;
     int block0, nbx;
+    const unsigned *amax;   // FP16X3: the operand's row maxima (row scales), else null
+};
+// FP16X3 grouped plans: the row-maximum jobs (out = maxima) and the maxima
+// buffer of all problems, zeroed before the pass.
+struct GroupedScales {
+    unsigned *amax;
+    long count;
+    const PrepJob *jobs;
+    int njobs;
+    long blocks;
 };
 
 // One row-major GEMM after major/transpose normalisation:
@@ -105,7 +116,8 @@ int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
 // the preparation (jobs) then the core (one block per tile).
 int mfma_grouped_tile(int mode, int variant, int *bm, int *bn);
 int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
-                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s);
+                        const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s,
+                        const GroupedScales *scales = nullptr);
 // Convolution in a split form: g.A = weights (g.M x g.K, k-contiguous), g.B
 // unused -- the im2col operand of cd is gathered into planes.
 int launch_mfma_conv(const Gemm<float> &g, const ConvDesc &cd, int mode, int variant, hipStream_t s);
