@@ -536,9 +536,62 @@ def cpu_baseline(n, seconds_target):
                 break
     except OSError:
         pass
-    return {"value": round(tflops, 6), "unit": "TFLOP/s", "cores": 1, "kind": kind,
-            "sample": f"{fname}, R/N/N, alpha=1.5 beta=0.5, a {band}-row band of C (M={band}, N=K={n}) of the "
-                      f"{n}^3 workload, {calls} calls in {t_total:.1f} s, 1 thread of {cpu} (nproc={os.cpu_count()})"}
+    res = {"value": round(tflops, 6), "unit": "TFLOP/s", "cores": 1, "kind": kind,
+           "sample": f"{fname}, R/N/N, alpha=1.5 beta=0.5, a {band}-row band of C (M={band}, N=K={n}) of the "
+                     f"{n}^3 workload, {calls} calls in {t_total:.1f} s, 1 thread of {cpu} (nproc={os.cpu_count()})"}
+    if L is not None:
+        res["all_cores"] = cpu_all_cores(L, n, A, B, seconds_target / 2, cpu)
+        res["dgemm"] = cpu_dgemm(L, 4096, seconds_target / 2, cpu)
+    return res
+
+
+def cpu_all_cores(L, n, A, B, seconds_target, cpu):
+    """Every host core at once on the n^3 workload: each thread runs the
+    reference's sgemm_avx on its own 256-row band of C (sgemm_avx keeps its
+    scratch on the stack, sgemm_avx256.h:334, so disjoint bands are safe;
+    ctypes drops the GIL during the call).  The threading is this build's
+    arrangement, not the reference's (its harness is single-threaded)."""
+    import concurrent.futures as cf
+    threads = max(1, min(16, os.cpu_count() or 1))
+    band = A.shape[0]
+    g = np.random.Generator(np.random.PCG64(8))
+    Cs = [uniform(g, (band, n)) for _ in range(threads)]
+
+    def one(i):
+        L.sgemm_avx(b"R", b"N", b"N", band, n, n, 1.5, A.ctypes.data, n, B.ctypes.data, n, 0.5, Cs[i].ctypes.data, n)
+
+    rounds, t_total = 0, 0.0
+    with cf.ThreadPoolExecutor(threads) as ex:
+        while t_total < seconds_target and rounds < 16:
+            t0 = time.perf_counter()
+            list(ex.map(one, range(threads)))
+            t_total += time.perf_counter() - t0
+            rounds += 1
+    tflops = 2.0 * band * n * n * threads * rounds / t_total / 1e12
+    return {"value": round(tflops, 6), "unit": "TFLOP/s", "cores": threads, "kind": "reference",
+            "sample": f"sgemm_avx (sgemm_avx256.h:392) on {threads} threads, each a {band}-row band of C of the "
+                      f"{n}^3 workload, {rounds} rounds in {t_total:.1f} s, {cpu} (nproc={os.cpu_count()}); "
+                      "threading is this build's arrangement"}
+
+
+def cpu_dgemm(L, n, seconds_target, cpu):
+    """The reference's dgemm_avx (dgemm_avx.h:844-923) on a row band of the
+    n^3 FP64 workload (R/N/N, alpha = 1, beta = 0, normal(0,1) inputs), one
+    thread: dgemm_avx packs into static buffers, so it is not threaded."""
+    band = 128
+    g = np.random.Generator(np.random.PCG64(9))
+    A, B = g.standard_normal((band, n)), g.standard_normal((n, n))
+    C = np.zeros((band, n))
+    calls, t_total = 0, 0.0
+    while t_total < seconds_target and calls < 64:
+        t0 = time.perf_counter()
+        L.dgemm_avx(b"R", b"N", b"N", band, n, n, 1.0, A.ctypes.data, n, B.ctypes.data, n, 0.0, C.ctypes.data, n)
+        t_total += time.perf_counter() - t0
+        calls += 1
+    tflops = 2.0 * band * n * n * calls / t_total / 1e12
+    return {"value": round(tflops, 6), "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+            "sample": f"dgemm_avx (dgemm_avx.h:844), R/N/N, a {band}-row band of C (M={band}, N=K={n}) of the "
+                      f"{n}^3 FP64 workload, {calls} calls in {t_total:.1f} s, 1 thread of {cpu}"}
 
 
 def load_traffic(form):
