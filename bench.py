@@ -5,7 +5,8 @@ Headline workload: SGEMM M = N = K = 8192, row-major NN, alpha = 1.5,
 beta = 0.5, A, B, C ~ U[-1, 1) (24-bit, seeded), inputs resident in HBM.
 A "step" is one call of the headline form, the split-BF16 x3 SGEMM on MFMA;
 the plain FP32 kernel and the other forms (FP16x3 with per-row scales,
-BF16x6, BF16x1, FP8 block-scaled) are timed the same way beside it.  Timing: W untimed warm-up calls, then K
+FP32 on the FP32 matrix cores, BF16x6, BF16x1, FP8 block-scaled) are timed
+the same way beside it.  Timing: W untimed warm-up calls, then K
 calls bracketed by device synchronisation and HIP events on the launch stream.
 The dominant kernel's own average duration (the MFMA core, without the
 operand-preparation kernels) comes from the library's event hook
@@ -24,7 +25,8 @@ Also reported (one JSON line in total):
   * host-pointer calls with copies included (sgemm_test.c's workload, and
     one 8192^3 uut call);
   * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
-    into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread.
+    into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread;
+    beside it the same on every host core, and dgemm_avx on one thread.
 """
 import argparse
 import json
@@ -42,14 +44,14 @@ import ugemm_amd as ug  # noqa: E402
 from ugemm_amd import hip  # noqa: E402
 
 # MI355X published dense peaks (TFLOP/s) and HBM bandwidth.
-PEAK = {"fp32_vector": 157.3, "bf16_mfma": 2516.6, "fp16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}
+PEAK = {"fp32_vector": 157.3, "fp32_mfma": 157.3, "bf16_mfma": 2516.6, "fp16_mfma": 2516.6, "fp8_mfma": 5033.2, "fp64": 78.6}
 HBM_GBPS = 8000.0
 # Form -> (peak used for roofline.frac, MFMA products per useful product)
 FORM_PEAK = {"fp32": ("fp32_vector", 1), "bf16x3": ("bf16_mfma", 3), "bf16x6": ("bf16_mfma", 6),
-             "fp16x3": ("fp16_mfma", 3),
+             "fp16x3": ("fp16_mfma", 3), "fp32_mfma": ("fp32_mfma", 1),
              "bf16x1": ("bf16_mfma", 1), "fp8": ("fp8_mfma", 1)}
 CORE_KERNEL = {"fp32": "sgemm_fp32_kernel", "bf16x3": "split_nt_core", "bf16x6": "split_nt_core",
-               "fp16x3": "split_nt_core",
+               "fp16x3": "split_nt_core", "fp32_mfma": "sgemm_mfma32_kernel",
                "bf16x1": "split_nt_core", "fp8": "fp8_scaled_core"}
 
 
@@ -157,7 +159,7 @@ def config3(stream, steps, warmup):
         exact = A[rows].astype(np.float64) @ B.astype(np.float64)
         scale = np.abs(A[rows].astype(np.float64)) @ np.abs(B.astype(np.float64))
         res[label] = {}
-        for name in ("fp32", "bf16x3", "fp16x3", "bf16x6"):
+        for name in ("fp32", "fp32_mfma", "bf16x3", "fp16x3", "bf16x6"):
             mid = ug.MODES[name]
 
             def call():
@@ -610,7 +612,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
-    ap.add_argument("--forms", default="bf16x3,fp32,fp16x3,bf16x6,bf16x1,fp8")
+    ap.add_argument("--forms", default="bf16x3,fp32,fp16x3,fp32_mfma,bf16x6,bf16x1,fp8")
     ap.add_argument("--headline", default="bf16x3")
     ap.add_argument("--quick", action="store_true", help="headline forms only")
     ap.add_argument("--no-cpu", action="store_true")
@@ -676,6 +678,11 @@ def main():
                 "met": bool(v["tflops"] >= 3 * p["tflops"] and v["max_rel_err"] <= p["max_rel_err"]
                             and v["mean_rel_err"] <= p["mean_rel_err"])}
             for k, v in res.items() if k != "fp32" and "tflops" in v and "max_rel_err" in v}
+        # The same ratio against the FP32-MFMA form, the fastest FP32-class
+        # kernel here (not the plain vector-FMA kernel the target names).
+        if "fp32_mfma" in res:
+            line["speedup_vs_fp32_mfma"] = {k: round(v["tflops"] / res["fp32_mfma"]["tflops"], 3)
+                                            for k, v in res.items() if "tflops" in v}
     if args.calibrate_copy:
         nbytes = 1 << 30
         src, dst = hip.DeviceArray(nbytes // 4, np.float32), hip.DeviceArray(nbytes // 4, np.float32)
@@ -696,7 +703,8 @@ def main():
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)
         line["config5_precision_ladder"] = sgemm_forms(n, A5, B5, C5, 1.0, 0.0, ["fp32", "bf16x3", "fp16x3",
-                                                                                  "bf16x6", "bf16x1", "fp8"],
+                                                                                  "fp32_mfma", "bf16x6", "bf16x1",
+                                                                                  "fp8"],
                                                        max(5, args.steps // 4), 2, stream, rows, "config5")
         del A5, B5, C5
     if not args.no_cpu:

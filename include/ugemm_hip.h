@@ -48,7 +48,9 @@ typedef enum {
     UGEMM_MODE_BF16X6 = 4,    /* 3-way BF16 split, six MFMA products: FP32-level error at large K */
     UGEMM_MODE_FP16X3 = 5,    /* 2-way FP16 split with a power-of-two scale per row of op(A) and column
                                  of op(B), hi*hi + hi*lo + lo*hi on MFMA: FP32-level error, 3 products */
-    UGEMM_MODE_COUNT = 6
+    UGEMM_MODE_FP32_MFMA = 6, /* FP32 operands and accumulation on the FP32 matrix cores: FP32 error class,
+                                 no operand preparation (no counterpart in the reference) */
+    UGEMM_MODE_COUNT = 7
 } ugemm_mode_t;
 
 /* ------------------------------------------------------------------ */
@@ -57,7 +59,7 @@ typedef enum {
 /*    (check_sgemm.c:87-103, check_dgemm.c:86-102); same argument list  */
 /*    as sgemm_cpu/dgemm_cpu (ugemm.h:162-176, ugemm.h:287-301).       */
 /*    sgemm_hip runs UGEMM_MODE_FP32 unless UGEMM_HIP_MODE is set      */
-/*    (fp32|bf16x3|bf16x1|fp8|bf16x6|fp16x3).                          */
+/*    (fp32|bf16x3|bf16x1|fp8|bf16x6|fp16x3|fp32_mfma).                */
 /* ------------------------------------------------------------------ */
 void sgemm_hip(char major, char transa, char transb, int M, int N, int K,
                float alpha, const float *A, int lda, const float *B, int ldb,

@@ -20,10 +20,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libugemm_hip.so")
 
-MODE_FP32, MODE_BF16X3, MODE_BF16X1, MODE_FP8_BLOCK, MODE_BF16X6, MODE_FP16X3 = 0, 1, 2, 3, 4, 5
+MODE_FP32, MODE_BF16X3, MODE_BF16X1, MODE_FP8_BLOCK, MODE_BF16X6, MODE_FP16X3, MODE_FP32_MFMA = 0, 1, 2, 3, 4, 5, 6
 MODE_DGEMM = -1  # variant tables of the DGEMM family
 MODES = {"fp32": MODE_FP32, "bf16x3": MODE_BF16X3, "bf16x1": MODE_BF16X1,
-         "fp8": MODE_FP8_BLOCK, "bf16x6": MODE_BF16X6, "fp16x3": MODE_FP16X3}
+         "fp8": MODE_FP8_BLOCK, "bf16x6": MODE_BF16X6, "fp16x3": MODE_FP16X3, "fp32_mfma": MODE_FP32_MFMA}
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_UNSUPPORTED, ERR_TUNE_FILE = 0, -1, -2, -3, -4, -5, -6
 

@@ -107,6 +107,11 @@ int fp32_variant_count();
 const char *fp32_variant_name(int v);
 int fp32_pick_variant(const Gemm<float> &g);
 int launch_fp32(const Gemm<float> &g, int variant, hipStream_t s);
+// FP32 SGEMM on FP32 MFMA (sgemm_mfma32.hip).
+int m32_variant_count();
+const char *m32_variant_name(int v);
+int m32_pick_variant(const Gemm<float> &g);
+int launch_m32(const Gemm<float> &g, int variant, hipStream_t s);
 
 int mfma_variant_count(int mode);
 const char *mfma_variant_name(int mode, int v);

@@ -243,6 +243,7 @@ static int choose_variant(const Gemm<float> &g, int mode, int *splits) {
     if (v >= 0) return v;
     switch (mode) {
         case UGEMM_MODE_FP32: return fp32_pick_variant(g);
+        case UGEMM_MODE_FP32_MFMA: return m32_pick_variant(g);
         case UGEMM_MODE_FP8_BLOCK: return -1;   // launch_fp8 picks by grid size
         default: return mfma_pick_variant(g, mode);
     }
@@ -264,6 +265,7 @@ int run_sgemm(const Gemm<float> &g, int mode, int variant, hipStream_t s) {
     }
     switch (mode) {
         case UGEMM_MODE_FP32: return launch_fp32(gs, variant, s);
+        case UGEMM_MODE_FP32_MFMA: return launch_m32(gs, variant, s);
         case UGEMM_MODE_BF16X1:
         case UGEMM_MODE_BF16X3:
         case UGEMM_MODE_BF16X6:
@@ -435,6 +437,7 @@ static int env_mode() {
             if (!strcmp(e, "bf16x6")) return (int)UGEMM_MODE_BF16X6;
             if (!strcmp(e, "fp8")) return (int)UGEMM_MODE_FP8_BLOCK;
             if (!strcmp(e, "fp16x3")) return (int)UGEMM_MODE_FP16X3;
+            if (!strcmp(e, "fp32_mfma")) return (int)UGEMM_MODE_FP32_MFMA;
         }
         return (int)UGEMM_MODE_FP32;
     }();
@@ -467,7 +470,8 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
     int rc = conv_shape(ich, w, h, k, pad, stride, ch, &hcol, &wcol);
     if (rc) return rc;
     if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "conv: unknown mode %d", mode);
-    if (mode == UGEMM_MODE_FP8_BLOCK) return fail(UGEMM_ERR_UNSUPPORTED, "conv: the FP8 form is not provided");
+    if (mode == UGEMM_MODE_FP8_BLOCK || mode == UGEMM_MODE_FP32_MFMA)
+        return fail(UGEMM_ERR_UNSUPPORTED, "conv: mode %d is not provided", mode);
     if (!d_in || !d_weights || !d_out) return fail(UGEMM_ERR_ARG, "conv: NULL buffer");
     const int K = ich * k * k;
     void *ws = nullptr;
@@ -828,6 +832,7 @@ int ugemm_hip_timing_collect(double *total_ms) {
 int ugemm_hip_variant_count(int mode) {
     switch (mode) {
         case UGEMM_MODE_FP32: return fp32_variant_count();
+        case UGEMM_MODE_FP32_MFMA: return m32_variant_count();
         case UGEMM_MODE_FP8_BLOCK: return fp8_variant_count();
         case -1: return dgemm_variant_count();
         case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: case UGEMM_MODE_FP16X3:
@@ -839,6 +844,7 @@ int ugemm_hip_variant_count(int mode) {
 const char *ugemm_hip_variant_name(int mode, int v) {
     switch (mode) {
         case UGEMM_MODE_FP32: return fp32_variant_name(v);
+        case UGEMM_MODE_FP32_MFMA: return m32_variant_name(v);
         case UGEMM_MODE_FP8_BLOCK: return fp8_variant_name(v);
         case -1: return dgemm_variant_name(v);
         case UGEMM_MODE_BF16X1: case UGEMM_MODE_BF16X3: case UGEMM_MODE_BF16X6: case UGEMM_MODE_FP16X3:
