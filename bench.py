@@ -199,7 +199,7 @@ def config3(stream, steps, warmup):
     return res
 
 
-def config4(stream, fp32_single_tf):
+def config4(stream, fp32_single_tf, single_tf_by_form=None):
     """512 uneven im2col-shaped FP32 problems: M = 32*U{1..16}, N = H*W with
     H = W in {7, 14, 28, 56}, K = 9*16*U{1..32}; U[-1,1) inputs."""
     g = np.random.Generator(np.random.PCG64(0x55474D4D04))
@@ -255,11 +255,12 @@ def config4(stream, fp32_single_tf):
     e1.synchronize()
     ms_loop = e0.elapsed_ms(e1) / loop_reps
     plan.destroy()
-    # The same batch in the split forms: one preparation launch for every
-    # operand (FP16X3: after a row-maximum launch), one split-core block per
-    # tile (captured, tuned); each form's own bound on the same problems.
+    # The same batch in the split forms (one preparation launch for every
+    # operand -- FP16X3: after a row-maximum launch -- then one split-core
+    # block per tile) and on the FP32 matrix cores (the persistent kernel with
+    # FP32-MFMA tiles); captured, tuned; each form's own bound.
     split_legs = {}
-    for form, extra in (("bf16x3", 2.0 ** -14), ("fp16x3", 2.0 ** -19)):
+    for form, extra in (("bf16x3", 2.0 ** -14), ("fp16x3", 2.0 ** -19), ("fp32_mfma", 0.0)):
         plan_s = ug.GroupedPlan(probs, mode=ug.MODES[form], capture=True)
         tuned_s = plan_s.tune()
         for _ in range(2):
@@ -280,9 +281,12 @@ def config4(stream, fp32_single_tf):
             bound = (2 * (K + 1) * 2.0 ** -24 + extra) * (np.abs(A) @ np.abs(B))
             worst_s = max(worst_s, float(np.max(np.abs(got - A @ B) / bound)))
         plan_s.destroy()
-        split_legs[form + "_grouped"] = {"split_core_variant": ug.variants(ug.MODES[form])[tuned_s],
+        tf_s = flop / (ms_s * 1e-3) / 1e12
+        single = (single_tf_by_form or {}).get(form)
+        split_legs[form + "_grouped"] = {"variant": ug.grouped_variants(ug.MODES[form])[tuned_s],
                                          "grouped_graph_ms": round(ms_s, 4),
-                                         "grouped_tflops": round(flop / (ms_s * 1e-3) / 1e12, 3),
+                                         "grouped_tflops": round(tf_s, 3),
+                                         "frac_of_single_large_gemm_rate": round(tf_s / single, 4) if single else None,
                                          "checked_problems_worst_err_over_bound": worst_s}
     tf_group = flop / (ms_group * 1e-3) / 1e12
     r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
@@ -295,7 +299,8 @@ def config4(stream, fp32_single_tf):
     r.update(split_legs)
     log(f"[config4] grouped {tf_group:.1f} TFLOP/s ({ms_group:.3f} ms), loop {r['loop_plain_tflops']:.1f} TFLOP/s, "
         f"{r['frac_of_single_large_gemm_rate']} of single-GEMM rate; bf16x3 grouped "
-        f"{r['bf16x3_grouped']['grouped_tflops']:.1f}, fp16x3 grouped {r['fp16x3_grouped']['grouped_tflops']:.1f} TFLOP/s")
+        f"{r['bf16x3_grouped']['grouped_tflops']:.1f}, fp16x3 grouped {r['fp16x3_grouped']['grouped_tflops']:.1f}, "
+        f"fp32_mfma grouped {r['fp32_mfma_grouped']['grouped_tflops']:.1f} TFLOP/s")
     for d in (dpa, dpb, dC):
         d.free()
     e0.destroy()
@@ -694,7 +699,7 @@ def main():
     if not args.quick:
         line["config3"] = config3(stream, args.steps, args.warmup)
         fp32_tf = res.get("fp32", {}).get("tflops")
-        line["config4_batched"] = config4(stream, fp32_tf)
+        line["config4_batched"] = config4(stream, fp32_tf, {k: v.get("tflops") for k, v in res.items()})
         line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
         line["split_k"] = split_k(stream)
         line["blas12"] = blas12(stream)

@@ -180,6 +180,13 @@ int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode
  * count pins one for the plans created next on the thread. */
 int ugemm_hip_grouped_variant_count(void);
 const char *ugemm_hip_grouped_variant_name(int variant);
+/* The same per plan mode: UGEMM_MODE_FP32 (the variants above),
+ * UGEMM_MODE_FP32_MFMA (the same five tile choices on the FP32 matrix
+ * cores) and the split forms (their split-core variants, as
+ * ugemm_hip_variant_name).  Count: UGEMM_ERR_ARG for a mode grouped plans
+ * do not provide; name: NULL out of range. */
+int ugemm_hip_grouped_mode_variant_count(int mode);
+const char *ugemm_hip_grouped_mode_variant_name(int mode, int variant);
 
 /* ------------------------------------------------------------------ */
 /* 5. BLAS-1/2 helpers (HBM-bound kernels).                           */

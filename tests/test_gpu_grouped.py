@@ -165,7 +165,7 @@ def test_grouped_split_forms(mode, variant):
     plan.destroy()
 
 
-def test_grouped_split_form_graph_replays_and_tuning():
+def test_grouped_split_form_graph_replays_and_tuning(tmp_path):
     """A captured BF16x3 plan replays bitwise identically; tuning picks one
     of the split-core variants and leaves results within the bound."""
     mode = ug.MODE_BF16X3
@@ -186,6 +186,75 @@ def test_grouped_split_form_graph_replays_and_tuning():
         ok, worst = d.case.judge(C, mode=mode)
         assert ok, worst
     plan.destroy()
+    # The tuned split-core variant (up to 6) survives a save / load round trip.
+    L = ug.load()
+    path = str(tmp_path / "tune.txt").encode()
+    assert L.ugemm_hip_tune_save(path) == ug.OK
+    rows = [l.split() for l in open(path.decode()).read().split("\n")[1:] if l.strip()]
+    assert L.ugemm_hip_tune_load(path) == len(rows)
+    assert any(int(r[0]) == 16 and int(r[2]) == mode and int(r[6]) == v for r in rows)   # 16: grouped key
+
+
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4])
+def test_grouped_fp32_mfma(variant):
+    """FP32-MFMA plans: the grouped persistent kernel with FP32 matrix-core
+    tiles (variant 4: 128x128 or 64x128 per problem); every problem of the
+    uneven batch (all layouts, unaligned leading dimensions, the tiny / K = 0
+    / alpha = 0 edge problems) meets the plain FP32 bound."""
+    mode = ug.MODE_FP32_MFMA
+    cases = [DevCase(c) for c in uneven_cases(16, 70 + variant)]
+    ug.force_variant(mode, variant)
+    try:
+        plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=False)
+    finally:
+        ug.force_variant(mode, -1)
+    plan.run(None)
+    hip.synchronize()
+    for d in cases:
+        ok, worst = d.case.judge(d.dC.download(), mode=mode)
+        assert ok, f"variant {variant} {d.case.M}x{d.case.N}x{d.case.K} {d.case.ta}{d.case.tb}: {worst:.3g}"
+    plan.destroy()
+
+
+def test_grouped_fp32_mfma_graph_replays_tuning_and_cache(tmp_path):
+    """A captured FP32-MFMA plan replays bitwise identically after tuning;
+    the tuning cache keeps the grouped entries of every form on reload."""
+    mode = ug.MODE_FP32_MFMA
+    names = ug.grouped_variants(mode)
+    assert len(names) == 5 and all(n.startswith("f32_mfma_") for n in names)
+    cases = [DevCase(c) for c in uneven_cases(20, 93, max_cin=128, sizes=(7, 14, 28, 56))]
+    plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
+    v = plan.tune()
+    assert 0 <= v < 5
+    outs = []
+    for _ in range(3):
+        for d in cases:
+            d.reset()
+        plan.run(None)
+        hip.synchronize()
+        outs.append([d.dC.download() for d in cases])
+    for run in outs[1:]:
+        for a, b in zip(outs[0], run):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for d, C in zip(cases, outs[0]):
+        ok, worst = d.case.judge(C, mode=mode)
+        assert ok, worst
+    plan.destroy()
+    L = ug.load()
+    path = str(tmp_path / "tune.txt").encode()
+    assert L.ugemm_hip_tune_save(path) == ug.OK
+    rows = [l for l in open(path.decode()).read().split("\n")[1:] if l.strip()]
+    assert L.ugemm_hip_tune_load(path) == len(rows)
+
+
+def test_grouped_mode_variant_tables():
+    L = ug.load()
+    assert ug.grouped_variants(ug.MODE_FP32) == ug.grouped_variants()
+    for mode in (ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3):
+        assert ug.grouped_variants(mode) == ug.variants(mode)
+    for mode in (ug.MODE_FP8_BLOCK, 42, -1):
+        assert L.ugemm_hip_grouped_mode_variant_count(mode) == ug.ERR_ARG
+    assert L.ugemm_hip_grouped_mode_variant_name(ug.MODE_FP32_MFMA, 5) is None
 
 
 # A forced single-GEMM variant the grouped kernel lacks (5) is ignored.

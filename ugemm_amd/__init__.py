@@ -89,6 +89,8 @@ def load(path=None):
         "ugemm_hip_tune_grouped": (i, [vp]),
         "ugemm_hip_grouped_variant_count": (i, []),
         "ugemm_hip_grouped_variant_name": (cs, [i]),
+        "ugemm_hip_grouped_mode_variant_count": (i, [i]),
+        "ugemm_hip_grouped_mode_variant_name": (cs, [i, i]),
         "ugemm_hip_tune_load": (i, [cs]),
         "ugemm_hip_tune_save": (i, [cs]),
         "ugemm_hip_tune_lookup": (i, [c, c, i, i, i, i]),
@@ -255,10 +257,16 @@ def variants(mode):
     return [L.ugemm_hip_variant_name(mode, v).decode() for v in range(L.ugemm_hip_variant_count(mode))]
 
 
-def grouped_variants():
-    """Names of the grouped kernel's tile variants (indices of ugemm_hip_tune_grouped)."""
+def grouped_variants(mode=None):
+    """Names of the grouped kernel's tile variants (indices of ugemm_hip_tune_grouped);
+    with a mode, those of a plan of that mode."""
     L = load()
-    return [L.ugemm_hip_grouped_variant_name(v).decode() for v in range(L.ugemm_hip_grouped_variant_count())]
+    if mode is None:
+        return [L.ugemm_hip_grouped_variant_name(v).decode() for v in range(L.ugemm_hip_grouped_variant_count())]
+    n = L.ugemm_hip_grouped_mode_variant_count(mode)
+    if n < 0:
+        raise UgemmError(n, f"grouped plans do not provide mode {mode}")
+    return [L.ugemm_hip_grouped_mode_variant_name(mode, v).decode() for v in range(n)]
 
 
 def force_variant(mode, variant):

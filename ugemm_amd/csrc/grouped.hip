@@ -29,6 +29,7 @@
 // sgemm_ocl1.h:271-300: M = output channels, N = output pixels, K = k*k*Cin).
 #include "internal.h"
 #include "fp32_tile.h"
+#include "mfma32_tile.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -64,43 +65,71 @@ __global__ void grouped_reset_heads(int *heads) {
 template <class CF, bool CONV>
 __device__ __forceinline__ void grouped_tile(const DevProblem &p, int m0, int n0, float *lds) {
     const bool aligned = (p.layout & 4) != 0;
-    if constexpr (CONV) {
+    if constexpr (IsM32<CF>::value) {   // FP32-MFMA plans: GEMM problems only
+        static_assert(!CONV, "FP32-MFMA tiles take no implicit im2col operand");
+        switch (p.layout & 3) {
+            case 0:
+                m32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                              n0, lds, aligned);
+                break;
+            case 1:
+                m32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                             n0, lds, aligned);
+                break;
+            case 2:
+                m32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                             n0, lds, aligned);
+                break;
+            default:
+                m32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                            n0, lds, aligned);
+                break;
+        }
+    } else if constexpr (CONV) {
         fp32_tile<CF, false, false, 4, true>(p.M, p.N, p.K, p.alpha, p.A, p.lda, nullptr, 0, p.beta, p.C, p.ldc, m0,
                                              n0, lds, aligned, p.epi, p.cd);
-        return;
-    }
-    switch (p.layout & 3) {
-        case 0:
-            fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
-                                           lds, aligned);
-            break;
-        case 1:
-            fp32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
-                                          lds, aligned);
-            break;
-        case 2:
-            fp32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
-                                          lds, aligned);
-            break;
-        default:
-            fp32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0, n0,
-                                         lds, aligned);
-            break;
+    } else {
+        switch (p.layout & 3) {
+            case 0:
+                fp32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                               n0, lds, aligned);
+                break;
+            case 1:
+                fp32_tile<CF, true, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                              n0, lds, aligned);
+                break;
+            case 2:
+                fp32_tile<CF, false, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                              n0, lds, aligned);
+                break;
+            default:
+                fp32_tile<CF, true, true, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
+                                             n0, lds, aligned);
+                break;
+        }
     }
 }
 
 template <class CF>
 constexpr int lds_floats() {
-    return 2 * CF::BK * (CF::LDA + CF::LDB);
+    if constexpr (IsM32<CF>::value) return CF::kLdsFloats;
+    else return 2 * CF::BK * (CF::LDA + CF::LDB);
 }
 
-// At least 3 waves per SIMD (<= 168 VGPRs): the layout switch would
-// otherwise push the 128 x 128 variant just past that limit.
+// Waves per SIMD the persistent kernel is compiled for (register budget).
+template <class CF>
+constexpr int grouped_waves() {
+    return IsM32<CF>::value ? 2 : 3;
+}
+
+// At least 3 waves per SIMD for the FP32 vector tiles (<= 168 VGPRs): the
+// layout switch would otherwise push the 128 x 128 variant just past that
+// limit; 2 for the FP32-MFMA tiles.
 // CONV: a plan of convolutions (every problem has layout bit 3), else of GEMMs.
 // CS: the tile of a mixed plan whose DevTile::shape is 1 (CF for shape 0);
 // the same as CF for a single-shape plan.  Both shapes have CF::NT threads.
 template <class CF, bool CONV, class CS = CF>
-__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(3))) void grouped_fp32_kernel(
+__global__ __launch_bounds__(CF::NT) __attribute__((amdgpu_waves_per_eu(grouped_waves<CF>()))) void grouped_fp32_kernel(
     const DevProblem *__restrict__ probs, const DevTile *__restrict__ tiles, const int *__restrict__ q_begin,
     int *heads) {
     static_assert(CS::NT == CF::NT, "the shapes of a mixed plan share the workgroup size");
@@ -138,6 +167,12 @@ using G3 = Fp32Cfg<64, 128, 16, 4, 8>;
 // 64 x 128 tile (G3), whichever covers its rows with less padding (the
 // larger, more efficient tile when it pads at most 10% more).
 constexpr int kNumVariants = 5, kMixed = 4;
+// The same five tile choices on the FP32 matrix cores (UGEMM_MODE_FP32_MFMA
+// plans), 4 waves each; variant 4 mixes GM0 and GM3 per problem.
+using GM0 = M32Cfg<128, 128, 16, 2, 2>;
+using GM1 = M32Cfg<64, 64, 16, 2, 2>;
+using GM2 = M32Cfg<128, 64, 16, 2, 2>;
+using GM3 = M32Cfg<64, 128, 16, 2, 2>;
 
 struct VariantOps {
     int bm, bn, bk;          // shape 0
@@ -148,11 +183,24 @@ struct VariantOps {
 
 template <class CF, class CS = CF>
 VariantOps ops_of() {
-    return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
-                      (const void *)grouped_fp32_kernel<CF, true, CS>, CF::NT};
+    if constexpr (IsM32<CF>::value)
+        return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
+                          nullptr, CF::NT};
+    else
+        return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
+                          (const void *)grouped_fp32_kernel<CF, true, CS>, CF::NT};
 }
 
-VariantOps variant_ops(int v) {
+VariantOps variant_ops(int mode, int v) {
+    if (mode == UGEMM_MODE_FP32_MFMA) {
+        switch (v) {
+            case 0: return ops_of<GM0>();
+            case 1: return ops_of<GM1>();
+            case 2: return ops_of<GM2>();
+            case 3: return ops_of<GM3>();
+            default: return ops_of<GM0, GM3>();
+        }
+    }
     switch (v) {
         case 0: return ops_of<G0>();
         case 1: return ops_of<G1>();
@@ -164,6 +212,8 @@ VariantOps variant_ops(int v) {
 
 const char *kGroupedNames[] = {"fp32_128x128x16_t8x8", "fp32_64x64x16_t4x4", "fp32_128x64x16_t8x4",
                                "fp32_64x128x16_t4x8", "fp32_mixed_128x128_or_64x128_per_problem"};
+const char *kGroupedM32Names[] = {"f32_mfma_128x128x16_w2x2", "f32_mfma_64x64x16_w2x2", "f32_mfma_128x64x16_w2x2",
+                                  "f32_mfma_64x128x16_w2x2", "f32_mfma_mixed_128x128_or_64x128_per_problem"};
 
 // Tile shape of problem rows M in a mixed plan: 0 (BM0 rows) unless its
 // padded rows exceed those of shape 1 (BM1 rows) by more than 10%.
@@ -174,7 +224,13 @@ int mixed_shape(int M, int bm0, int bm1) {
 
 }  // namespace
 
-int grouped_variant_count() { return kNumVariants; }
+int grouped_variant_count(int mode) {
+    if (mode == UGEMM_MODE_FP32 || mode == UGEMM_MODE_FP32_MFMA) return kNumVariants;
+    if (mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6 ||
+        mode == UGEMM_MODE_FP16X3)
+        return mfma_variant_count(mode);
+    return 0;
+}
 
 }  // namespace ugemm
 
@@ -247,7 +303,7 @@ int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
         return launch_mfma_grouped(p->mode, p->variant, p->d_jobs, p->njobs, p->prep_blocks, p->d_sprobs,
                                    p->d_stiles, p->ntiles, s, p->d_amax ? &sc : nullptr);
     }
-    const VariantOps ops = variant_ops(p->variant);
+    const VariantOps ops = variant_ops(p->mode, p->variant);
     hipLaunchKernelGGL(grouped_reset_heads, dim3(1), dim3(64), 0, s, p->d_heads);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "grouped queue reset");
@@ -266,7 +322,7 @@ int capture_plan(ugemm_grouped_plan *p);   // records the plan's launches in its
 int build_schedule(ugemm_grouped_plan *p, int v) {
     release_schedule(p);
     p->variant = v;
-    const VariantOps ops = variant_ops(v);
+    const VariantOps ops = variant_ops(p->mode, v);
     struct HostTile {
         DevTile t;
         long cost;
@@ -536,12 +592,11 @@ int finish_plan(ugemm_grouped_plan *p, int count, int capture, ugemm_grouped_pla
     }
     // A forced single-GEMM variant applies when the grouped kernel has it too
     // (split forms: every split-core variant).
-    const int nv = p->split ? mfma_variant_count(p->mode) : kNumVariants;
+    const int nv = grouped_variant_count(p->mode);
     int v = forced_variant(p->mode);
     if (v >= nv) v = -1;
     if (v < 0)
-        v = tuned_variant(TUNE_KEY_GROUPED, p->conv, p->split ? p->mode : 0, count, (int)std::min(p->flops / 1e6, 2e9),
-                          0);
+        v = tuned_variant(TUNE_KEY_GROUPED, p->conv, p->mode, count, (int)std::min(p->flops / 1e6, 2e9), 0);
     if (v < 0 || v >= nv) v = p->split ? default_split_variant(p) : default_variant(p);
     const int rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
     if (rc) {
@@ -563,8 +618,9 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
     if (count < 0 || (count > 0 && !problems)) return fail(UGEMM_ERR_ARG, "grouped: bad problem list");
     const bool split = mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6 ||
                        mode == UGEMM_MODE_FP16X3;
-    if (mode != UGEMM_MODE_FP32 && !split)
-        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32 and the split forms)", mode);
+    if (mode != UGEMM_MODE_FP32 && mode != UGEMM_MODE_FP32_MFMA && !split)
+        return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided (FP32, FP32-MFMA and the split forms)",
+                    mode);
     int rc = ensure_device();
     if (rc) return rc;
     ugemm_grouped_plan *p = new ugemm_grouped_plan();
@@ -702,6 +758,18 @@ int ugemm_hip_grouped_variant_count(void) { return kNumVariants; }
 
 const char *ugemm_hip_grouped_variant_name(int v) { return (v >= 0 && v < kNumVariants) ? kGroupedNames[v] : nullptr; }
 
+int ugemm_hip_grouped_mode_variant_count(int mode) {
+    const int n = grouped_variant_count(mode);
+    return n > 0 ? n : fail(UGEMM_ERR_ARG, "grouped: mode %d not provided", mode);
+}
+
+const char *ugemm_hip_grouped_mode_variant_name(int mode, int v) {
+    if (v < 0 || v >= grouped_variant_count(mode)) return nullptr;
+    if (mode == UGEMM_MODE_FP32) return kGroupedNames[v];
+    if (mode == UGEMM_MODE_FP32_MFMA) return kGroupedM32Names[v];
+    return mfma_variant_name(mode, v);
+}
+
 // Times each tile variant on the plan's own problems (C is overwritten, as by
 // a run) and keeps the fastest; the choice is remembered for plans with the
 // same problem count, work bucket and form.  Returns the chosen variant
@@ -717,7 +785,7 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(UGEMM_ERR_HIP, "events");
     const bool cap = p->capture;
     p->capture = false;
-    const int nv = p->split ? mfma_variant_count(p->mode) : kNumVariants;
+    const int nv = grouped_variant_count(p->mode);
     for (int v = 0; v < nv && !rc; ++v) {
         rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
         if (rc) break;
@@ -743,7 +811,7 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     hipStreamSynchronize(s);
     hipStreamDestroy(s);
     if (rc) return rc;
-    tune_record(TUNE_KEY_GROUPED, p->conv, p->split ? p->mode : 0, (int)p->probs.size(),
+    tune_record(TUNE_KEY_GROUPED, p->conv, p->mode, (int)p->probs.size(),
                 (int)std::min(p->flops / 1e6, 2e9), 0, best, best_ms / 3);
     return best;
 }

@@ -170,11 +170,12 @@ int launch_splitk_reduce_f32(int M, int N, int splits, float alpha, const float 
 int launch_splitk_reduce_f64(int M, int N, int splits, double alpha, const double *ws, double beta, double *C,
                              long ldc, hipStream_t s);
 
-// Tile variants of the grouped kernel (grouped.hip).
-int grouped_variant_count();
+// Tile variants of a grouped plan of `mode` (grouped.hip); 0 for a mode
+// grouped plans do not provide.
+int grouped_variant_count(int mode);
 
 // ---- tuning (tune.hip) ----
-// Tuning-table key of grouped plans (variants = grouped_variant_count()).
+// Tuning-table key of grouped plans (variants = grouped_variant_count(mode)).
 constexpr int TUNE_KEY_GROUPED = 16;
 // ta, tb: layout flags (grouped keys: convolution flag, split form).
 int tuned_variant(int mode, int ta, int tb, int M, int N, int K, int *splits = nullptr);   // -1 if none

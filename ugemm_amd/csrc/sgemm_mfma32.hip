@@ -4,180 +4,41 @@
 // plain FP32 kernel.  MI355X's FP32 MFMA peak (157.3 TF/s) equals its FP32
 // vector peak, but the matrix core is fed one register per operand per 2048
 // flop instead of one per 2 flop, so it runs much closer to that peak.
-//
-// A workgroup of WGM x WGN waves computes a BM x BN tile of C; each wave owns
-// a (BM/WGM) x (BN/WGN) grid of 16x16 fragments.  Operand tiles of BK move
-// global -> registers -> LDS, two LDS stages, one barrier per BK step.  Each
-// operand keeps its global orientation in LDS, so every LDS store is a
-// 16-byte vector store (no transposing writes):
-//   k-contiguous operand (A when op(A) = A, B when op(B) = B^T): L[row][k],
-//     row length BK + 4 floats; a lane reads its four k values as one
-//     16-byte ds_read;
-//   row-contiguous operand: L[k][row], row length rows + 4 floats; a lane
-//     reads four single floats, 16 lanes over 64 contiguous bytes.
-// The 16x16x4 MFMA takes from lane l the element (row l%16, k l/16).  Within
-// a 16-deep k slab the lane's k group g = l/16 supplies k = 4g + q to the
-// q-th MFMA (q = 0..3): both operands use this same permutation of k, so
-// the sum over the slab is unchanged, and a lane's four k values are
-// adjacent in memory for the k-contiguous layout.  The D fragment holds rows
-// 4*(l/16) + r, column l%16.
+// The tile computation (LDS layout, k-permuted fragments) is m32_tile in
+// mfma32_tile.h.
 //
 // Split-K (gridDim.y > 1): block (tile, z) multiplies K columns
 // [z*kc, min(K, (z+1)*kc)) into the partial slice ws + z*M*N; the partials
 // are added in fixed order by the split-K reduction, which also applies
 // alpha, beta and the epilogue.  Repeated calls are bitwise identical.
-// Semantics as sgemm_cpu (ugemm.h:287-411) with the BLAS beta == 0 rule
-// (C is not read), as gemm_cpu.h:119-123.
 #include "internal.h"
 #include "kernel_common.h"
+#include "mfma32_tile.h"
 
 namespace ugemm {
 namespace {
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-template <int BM_, int BN_, int BK_, int WGM_, int WGN_, int MINB_ = 1>
-struct M32Cfg {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, WGM = WGM_, WGN = WGN_, MINB = MINB_;
-    static constexpr int NT = 64 * WGM * WGN;
-    static constexpr int WTM = BM / WGM, WTN = BN / WGN, FM = WTM / 16, FN = WTN / 16;
-    static_assert(BK % 16 == 0 && WTM % 16 == 0 && WTN % 16 == 0, "fragment-aligned tile");
-    // LDS row length and stage size (floats) of an operand tile of ROWS rows.
-    template <bool KC, int ROWS>
-    static constexpr int ld() { return KC ? BK + 4 : ROWS + 4; }
-    template <bool KC, int ROWS>
-    static constexpr int stage() { return KC ? ROWS * (BK + 4) : BK * (ROWS + 4); }
-};
-
-// The lane's four k values (k .. k+3) of fragment row `row`.
-template <bool KC, int LD>
-__device__ __forceinline__ floatx4 frag4(const float *L, int row, int k) {
-    if constexpr (KC) {
-        return *reinterpret_cast<const floatx4 *>(L + row * LD + k);
-    } else {
-        return floatx4{L[k * LD + row], L[(k + 1) * LD + row], L[(k + 2) * LD + row], L[(k + 3) * LD + row]};
-    }
-}
 
 template <class CF, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(CF::NT, CF::MINB) void sgemm_mfma32_kernel(
     int M, int N, int K_, float alpha_, const float *__restrict__ A_, long lda, const float *__restrict__ B_,
     long ldb, float beta_, float *__restrict__ C_, long ldc_, int tiles_m, int tiles_n, Epilogue epi, int kc,
     float *__restrict__ ws) {
-    constexpr bool KCA = !TA, KCB = TB;
-    constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FM = CF::FM, FN = CF::FN;
-    constexpr int LDA = CF::template ld<KCA, BM>(), LDB = CF::template ld<KCB, BN>();
-    constexpr int SA = CF::template stage<KCA, BM>(), SB = CF::template stage<KCB, BN>();
-    __shared__ __attribute__((aligned(16))) float lds[2 * (SA + SB)];
-    float *As = lds, *Bs = lds + 2 * SA;
-
+    // This layout's two stages of both operands (at most CF::kLdsFloats).
+    constexpr int kLds = 2 * (CF::template stage<!TA, CF::BM>() + CF::template stage<TB, CF::BN>());
+    __shared__ __attribute__((aligned(16))) float lds[kLds];
     const bool part = gridDim.y > 1;
     const int kz0 = part ? (int)blockIdx.y * kc : 0;
     const int K = part ? min(kc, K_ - kz0) : K_;
-    const float *__restrict__ A = TA ? A_ + (long)kz0 * lda : A_ + kz0;
-    const float *__restrict__ B = TB ? B_ + kz0 : B_ + (long)kz0 * ldb;
-    float *__restrict__ C = part ? ws + (long)blockIdx.y * M * N : C_;
-    const long ldc = part ? (long)N : ldc_;
-    const float alpha = part ? 1.0f : alpha_, beta = part ? 0.0f : beta_;
-    const bool has_epi = !part && epi.active();
-
+    const float *A = TA ? A_ + (long)kz0 * lda : A_ + kz0;
+    const float *B = TB ? B_ + kz0 : B_ + (long)kz0 * ldb;
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = w / CF::WGN, wn = w % CF::WGN;
-
-    using ARegs = TileRegs<float, KCA ? BK : BM, KCA ? BM : BK, NT, VEC>;
-    using BRegs = TileRegs<float, KCB ? BK : BN, KCB ? BN : BK, NT, VEC>;
-    ARegs ra;
-    BRegs rb;
-    const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
-    const int ktiles = (K + BK - 1) / BK, kfull = K / BK;
-
-    auto load_tile = [&](int kt) {
-        const int k0 = kt * BK;
-        const float *a0 = TA ? A + (long)k0 * lda + m0 : A + (long)m0 * lda + k0;
-        const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
-        if (interior && kt < kfull) {
-            ra.template load<false>(a0, lda, t, 0, 0);
-            rb.template load<false>(b0, ldb, t, 0, 0);
-        } else {
-            const int mr = M - m0, nr = N - n0, kr = K - k0;
-            ra.template load<true>(a0, lda, t, KCA ? kr : mr, KCA ? mr : kr);
-            rb.template load<true>(b0, ldb, t, KCB ? kr : nr, KCB ? nr : kr);
-        }
-    };
-    auto store_tile = [&](int buf) {
-        ra.template store<true, LDA>(As + buf * SA, t);
-        rb.template store<true, LDB>(Bs + buf * SB, t);
-    };
-
-    floatx4 acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-
-    if (ktiles > 0) {
-        load_tile(0);
-        store_tile(0);
-    }
-    __syncthreads();
-
-    const int frow = lane & 15, g4 = (lane >> 4) * 4;
-    for (int kt = 0; kt < ktiles; ++kt) {
-        const int buf = kt & 1;
-        const bool more = kt + 1 < ktiles;
-        if (more) load_tile(kt + 1);
-        const float *as = As + buf * SA, *bs = Bs + buf * SB;
-#pragma unroll
-        for (int s = 0; s < BK / 16; ++s) {
-            floatx4 fa[FM], fb[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) fa[i] = frag4<KCA, LDA>(as, wm * CF::WTM + i * 16 + frow, s * 16 + g4);
-#pragma unroll
-            for (int j = 0; j < FN; ++j) fb[j] = frag4<KCB, LDB>(bs, wn * CF::WTN + j * 16 + frow, s * 16 + g4);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[i][q], fb[j][q], acc[i][j], 0, 0, 0);
-        }
-        if (more) store_tile(buf ^ 1);
-        __syncthreads();
-    }
-
-    // C values of one fragment row are loaded before any is stored (a load
-    // cannot pass a possibly aliasing store: one latency per row, not per element).
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const int mb = m0 + wm * CF::WTM + i * 16 + 4 * (lane >> 4);
-        float cv[FN][4];
-        if (beta != 0.0f) {
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    cv[j][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = mb + r;
-                if (interior || (m < M && n < N)) {
-                    float v = (beta == 0.0f) ? alpha * acc[i][j][r] : fmaf(alpha, acc[i][j][r], beta * cv[j][r]);
-                    if (has_epi) v = apply_epilogue(epi, v, m, n);
-                    C[(long)m * ldc + n] = v;
-                }
-            }
-        }
-    }
+    if (part)
+        m32_tile<CF, TA, TB, VEC>(M, N, K, 1.0f, A, lda, B, ldb, 0.0f, ws + (long)blockIdx.y * M * N, (long)N,
+                                  tm * CF::BM, tn * CF::BN, lds);
+    else
+        m32_tile<CF, TA, TB, VEC>(M, N, K, alpha_, A, lda, B, ldb, beta_, C_, ldc_, tm * CF::BM, tn * CF::BN, lds,
+                                  true, epi);
 }
 
 using V0 = M32Cfg<128, 128, 16, 2, 2, 2>;

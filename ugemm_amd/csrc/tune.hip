@@ -224,7 +224,7 @@ int ugemm_hip_tune_load(const char *path) {
         float ms;
         while (fscanf(f, "%d %d %d %d %d %d %d %f", &m, &a, &b, &lm, &ln, &lk, &v, &ms) == 8) {
             if (ver == 2 && fscanf(f, "%d", &sp) != 1) break;
-            const int nv = (m == TUNE_KEY_GROUPED) ? grouped_variant_count()
+            const int nv = (m == TUNE_KEY_GROUPED) ? grouped_variant_count(b)
                                                    : ((m < -1 || m >= UGEMM_MODE_COUNT) ? 0 : ugemm_hip_variant_count(m));
             if (v < 0 || v >= nv || sp < 0 || sp > 64) continue;
             std::lock_guard<std::mutex> lk2(g_mu);
