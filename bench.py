@@ -172,7 +172,13 @@ def config3(stream, steps, warmup):
             log(f"[config3] {label} {name}: {r['tflops']:.1f} TFLOP/s")
         for d in (dA, dB, dC):
             d.free()
-    # DGEMM 4096^3
+    res["dgemm_4096"] = dgemm_4096(stream, steps, g)
+    return res
+
+
+def dgemm_4096(stream, steps, g=None):
+    """DGEMM 4096^3 R/N/N, normal(0,1), alpha = 1, beta = 0, bit-compared across two runs."""
+    g = g if g is not None else np.random.Generator(np.random.PCG64(0x55474D4D03))
     n = 4096
     A = g.standard_normal((n, n))
     B = g.standard_normal((n, n))
@@ -192,11 +198,10 @@ def config3(stream, steps, warmup):
          "frac_of_fp64_peak": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12 / PEAK["fp64"], 4),
          "bitwise_reproducible": bool(np.array_equal(C1.view(np.uint64), C2.view(np.uint64)))}
     r.update(err_stats(C1[rows], exact, scale))
-    res["dgemm_4096"] = r
     log(f"[config3] dgemm 4096^3: {r['tflops']:.1f} TFLOP/s, reproducible {r['bitwise_reproducible']}")
     for d in (dA, dB, dC, dC2):
         d.free()
-    return res
+    return r
 
 
 def config4(stream, fp32_single_tf, single_tf_by_form=None):
@@ -611,6 +616,16 @@ def load_traffic(form):
         return None
 
 
+def load_counters(form):
+    """Counter figures of the form's core kernel from the committed rocprofv3
+    --pmc summary (counted FLOPs, shader clock under load, MFMA busy), if present."""
+    p = os.path.join(ROOT, "profiles", "flop_counters.json")
+    try:
+        return json.load(open(p)).get(form)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -620,6 +635,7 @@ def main():
     ap.add_argument("--forms", default="bf16x3,fp32,fp16x3,fp32_mfma,bf16x6,bf16x1,fp8")
     ap.add_argument("--headline", default="bf16x3")
     ap.add_argument("--quick", action="store_true", help="headline forms only")
+    ap.add_argument("--dgemm", action="store_true", help="with --quick: also the 4096^3 DGEMM")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -655,6 +671,14 @@ def main():
             "achieved_is": "algorithmic 2*M*N*K flop per launch / average core-kernel duration (HIP events)",
             "mfma_issue_frac": round(core_tf * mult / PEAK[peak_key], 4),
             "frac_of_fp32_vector_peak": round(core_tf / PEAK["fp32_vector"], 4)}
+    cnt = load_counters(head)
+    if cnt:
+        # From profiles/gemm_flop_counters.md: the clock the kernel runs at under
+        # load and the fraction of cycles its MFMA pipes are busy.
+        roof["counters"] = {"counted_flops_per_launch": cnt["flops"], "clock_ghz_under_load": cnt["clock_ghz"],
+                            "mfma_busy": cnt["mfma_busy"],
+                            "mfma_issue_frac_at_that_clock": round(core_tf * mult / (PEAK[peak_key] *
+                                                                                     cnt["clock_ghz"] / 2.4), 4)}
 
     line = {
         "metric": "SGEMM TFLOP/s at 8192^3: plain FP32 vs BF16x3-MFMA at FP32-level error",
@@ -696,6 +720,8 @@ def main():
         line["calibration_copy_bytes"] = nbytes
         src.free()
         dst.free()
+    if args.quick and args.dgemm:
+        line["dgemm_4096"] = dgemm_4096(stream, args.steps)
     if not args.quick:
         line["config3"] = config3(stream, args.steps, args.warmup)
         fp32_tf = res.get("fp32", {}).get("tflops")

@@ -295,23 +295,41 @@ __global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ 
 }
 
 // Row-contiguous source (X[k*ld + r]): block b covers columns r in
-// [256 * (b / kblocks), +256) and k in [256 * (b % kblocks), +256).
+// [256 * (b % cblocks), +256) and k in [256 * (b / cblocks), +256),
+// cblocks = ceil(rows / 256): consecutive blocks read neighbouring 1 KB row
+// segments (blocks that shared columns and differed only in k read addresses
+// a power-of-two multiple of the row pitch apart, which crowd the same HBM
+// channels).
 __device__ __forceinline__ void row_amax_rcont_block(const float *__restrict__ X, long ld, int rows, int K, bool vec,
                                                      long kblocks, unsigned *__restrict__ amax, long b) {
-    const int c0 = (int)(b / kblocks) * 256, k0 = (int)(b % kblocks) * 256;
+    const long cblocks = (rows + 255) / 256;
+    const int c0 = (int)(b % cblocks) * 256, k0 = (int)(b / cblocks) * 256;
+    (void)kblocks;
     const int cg = threadIdx.x & 63, kp = threadIdx.x >> 6;
     const int c = c0 + cg * 4;
     float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kk = kp; kk < 256 && k0 + kk < K; kk += 4) {
-        const float *p = X + (long)(k0 + kk) * ld + c;
-        if (vec && c + 4 <= rows) {
-            const float4 a = *reinterpret_cast<const float4 *>(p);
+    if (vec && c + 4 <= rows && k0 + 256 <= K) {
+        // Whole block: 64 independent 16-byte loads per thread, unrolled so
+        // that several are in flight (one at a time leaves HBM idle).
+        const float *p = X + (long)(k0 + kp) * ld + c;
+#pragma unroll 16
+        for (int kk = 0; kk < 64; ++kk) {
+            const float4 a = *reinterpret_cast<const float4 *>(p + (long)kk * 4 * ld);
             m[0] = fmaxf(m[0], fabsf(a.x)); m[1] = fmaxf(m[1], fabsf(a.y));
             m[2] = fmaxf(m[2], fabsf(a.z)); m[3] = fmaxf(m[3], fabsf(a.w));
-        } else {
+        }
+    } else {
+        for (int kk = kp; kk < 256 && k0 + kk < K; kk += 4) {
+            const float *p = X + (long)(k0 + kk) * ld + c;
+            if (vec && c + 4 <= rows) {
+                const float4 a = *reinterpret_cast<const float4 *>(p);
+                m[0] = fmaxf(m[0], fabsf(a.x)); m[1] = fmaxf(m[1], fabsf(a.y));
+                m[2] = fmaxf(m[2], fabsf(a.z)); m[3] = fmaxf(m[3], fabsf(a.w));
+            } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (c + j < rows) m[j] = fmaxf(m[j], fabsf(p[j]));
+                for (int j = 0; j < 4; ++j)
+                    if (c + j < rows) m[j] = fmaxf(m[j], fabsf(p[j]));
+            }
         }
     }
     __shared__ float sm[4][256];
