@@ -309,14 +309,19 @@ __device__ __forceinline__ void row_amax_rcont_block(const float *__restrict__ X
     const int c = c0 + cg * 4;
     float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (vec && c + 4 <= rows && k0 + 256 <= K) {
-        // Whole block: 64 independent 16-byte loads per thread, unrolled so
-        // that several are in flight (one at a time leaves HBM idle).
+        // Whole block: 64 independent 16-byte loads per thread, issued 16 at
+        // a time into registers before any is used (the compiler otherwise
+        // waits after every second load, which leaves HBM idle).
         const float *p = X + (long)(k0 + kp) * ld + c;
-#pragma unroll 16
-        for (int kk = 0; kk < 64; ++kk) {
-            const float4 a = *reinterpret_cast<const float4 *>(p + (long)kk * 4 * ld);
-            m[0] = fmaxf(m[0], fabsf(a.x)); m[1] = fmaxf(m[1], fabsf(a.y));
-            m[2] = fmaxf(m[2], fabsf(a.z)); m[3] = fmaxf(m[3], fabsf(a.w));
+        for (int kb = 0; kb < 64; kb += 16) {
+            float4 v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = *reinterpret_cast<const float4 *>(p + (long)(kb + i) * 4 * ld);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                m[0] = fmaxf(m[0], fabsf(v[i].x)); m[1] = fmaxf(m[1], fabsf(v[i].y));
+                m[2] = fmaxf(m[2], fabsf(v[i].z)); m[3] = fmaxf(m[3], fabsf(v[i].w));
+            }
         }
     } else {
         for (int kk = kp; kk < 256 && k0 + kk < K; kk += 4) {
@@ -955,8 +960,11 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
     }
     if (kcont) {
         const long total = (long)rows_pad * (k_pad / 8);
+        // One chunk per thread (the grid-stride loop only matters beyond 2^28
+        // chunks): a grid capped at 8192 blocks, four chunks per thread, ran
+        // 13% slower at 8192^3 (fewer loads in flight).
         long blocks = (total + 255) / 256;
-        if (blocks > 8192) blocks = 8192;
+        if (blocks > (1L << 20)) blocks = 1L << 20;
         hipLaunchKernelGGL((split_prep_kcont<NP, FMT>), dim3((unsigned)blocks), dim3(256), 0, s, X, ld, rows, K, vec,
                            out, plane_stride, rows_pad, k_pad, (const unsigned *)amax);
     } else {
