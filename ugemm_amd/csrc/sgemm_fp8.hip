@@ -59,20 +59,22 @@ template <bool KCONT>
 __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ X, long ld, int rows, int K, bool vec,
                                                       unsigned char *__restrict__ out, int rows_pad,
                                                       int *__restrict__ exps, int kblocks) {
-    __shared__ float red[256];
+    __shared__ float wmax[4];
     __shared__ __attribute__((aligned(16))) unsigned char tile[128][128 + 16];
     const int rb = blockIdx.x, kb = blockIdx.y;
     const int r0 = rb * SB, k0 = kb * SB;
     const int t = threadIdx.x;
-    // Each thread reads 64 elements: KCONT -> row t/2, k (t%2)*64..+63;
-    // else k t/2, rows (t%2)*64..+63 (coalesced along the contiguous index).
+    // The block's source is 128 lines (rows for KCONT, k for the other
+    // layout) of 128 contiguous floats.  Load q of a thread takes line
+    // q*8 + t/32, floats (t%32)*4 .. +3: each load instruction of a wave
+    // covers two whole 512-byte lines (coalesced), and the 16 loads are
+    // issued before any is used.
     float v[64];
-    const int a = t >> 1, b0 = (t & 1) * 64;
-    float amax = 0.0f;
+    const int cl = (t & 31) * 4, sl = t >> 5;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int c = b0 + q * 4;
-        const int r = KCONT ? r0 + a : r0 + c, k = KCONT ? k0 + c : k0 + a;
+        const int line = q * 8 + sl;
+        const int r = KCONT ? r0 + line : r0 + cl, k = KCONT ? k0 + cl : k0 + line;
         const float *p = KCONT ? X + (long)r * ld + k : X + (long)k * ld + r;
         const bool full = KCONT ? (r < rows && k + 4 <= K) : (k < K && r + 4 <= rows);
         if (full && vec) {
@@ -85,20 +87,19 @@ __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ 
                 v[q * 4 + j] = ok ? p[j] : 0.0f;
             }
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) amax = fmaxf(amax, fabsf(v[q * 4 + j])) ;
     }
+    float amax = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) amax = fmaxf(amax, fabsf(v[j]));
     // NaN/Inf make the block non-finite: keep it visible in amax.
 #pragma unroll
     for (int j = 0; j < 64; ++j)
         if (!__builtin_isfinite(v[j])) amax = __builtin_inff();
-    red[t] = amax;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
+    if ((t & 63) == 0) wmax[t >> 6] = amax;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) red[t] = fmaxf(red[t], red[t + s]);
-        __syncthreads();
-    }
-    const float bmax = red[0];
+    const float bmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
     const int e = block_exponent(bmax);
     const float inv = ldexpf(1.0f, -e);
     if (t == 0) exps[(long)rb * kblocks + kb] = e;
@@ -106,11 +107,12 @@ __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int w = pack4_e4m3(v[q * 4] * inv, v[q * 4 + 1] * inv, v[q * 4 + 2] * inv, v[q * 4 + 3] * inv);
+        const int line = q * 8 + sl;
         if (KCONT) {
-            *reinterpret_cast<int *>(&tile[a][b0 + q * 4]) = w;
+            *reinterpret_cast<int *>(&tile[line][cl]) = w;
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) tile[b0 + q * 4 + j][a] = (unsigned char)((unsigned)w >> (8 * j));
+            for (int j = 0; j < 4; ++j) tile[cl + j][line] = (unsigned char)((unsigned)w >> (8 * j));
         }
     }
     __syncthreads();
