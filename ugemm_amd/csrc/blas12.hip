@@ -127,8 +127,11 @@ int axpy(int N, T alpha, const T *x, int incx, T *y, int incy, hipStream_t s) {
     if (!x || !y) return fail(UGEMM_ERR_ARG, "axpy: NULL vector");
     const int cus = device_cu_count();
     if (incx == 1 && incy == 1 && ugemm_aligned16(x) && ugemm_aligned16(y)) {
+        // One 16-byte vector per thread (the grid-stride loop only matters
+        // beyond 2^28 vectors): a grid capped at 8 blocks per CU looped
+        // with two loads in flight per wave at a time.
         const long nv = N / (16 / (long)sizeof(T));
-        const long blocks = std::max<long>(1, std::min<long>((nv + 255) / 256, 8L * cus));
+        const long blocks = std::max<long>(1, std::min<long>((nv + 255) / 256, 1L << 20));
         hipLaunchKernelGGL(axpy_vec_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, (long)N, alpha, x, y);
     } else {
         const long blocks = std::max<long>(1, std::min<long>((N + 255L) / 256, 8L * cus));
