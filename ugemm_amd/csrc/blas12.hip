@@ -103,14 +103,18 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int M, int N, T alpha, cons
     const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (m >= M) return;
     const T *row = A + (long)m * lda;
-    T s0 = 0, s1 = 0;
+    // Four independent partial sums: four loads of the row (and of x) in
+    // flight per step; with two, the wave waited on HBM after each pair.
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int n = lane;
-    for (; n + 64 < N; n += 128) {
+    for (; n + 192 < N; n += 256) {
         s0 = fma(row[n], x[(long)n * incx], s0);
         s1 = fma(row[n + 64], x[(long)(n + 64) * incx], s1);
+        s2 = fma(row[n + 128], x[(long)(n + 128) * incx], s2);
+        s3 = fma(row[n + 192], x[(long)(n + 192) * incx], s3);
     }
-    if (n < N) s0 = fma(row[n], x[(long)n * incx], s0);
-    T s = s0 + s1;
+    for (; n < N; n += 64) s0 = fma(row[n], x[(long)n * incx], s0);
+    T s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) {
