@@ -330,7 +330,7 @@ def config4(stream, fp32_single_tf, single_tf_by_form=None):
     return r
 
 
-def config4_conv(stream, gemm_grouped_tf):
+def config4_conv(stream, gemm_grouped_tf, m32_grouped_tf=None):
     """Config 4 as convolutions: the same 512 draws (M = Cout, H = W, K = 9*Cin)
     as 3x3, pad 1, stride 1 convolutions with bias + LeakyReLU, im2col
     gathered inside the grouped kernel (no column matrix is stored).  The
@@ -355,21 +355,51 @@ def config4_conv(stream, gemm_grouped_tf):
     convs = [ug.ConvProblem(dpx.at(x_off[i]), int(Cin[i]), int(HW[i]), int(HW[i]), dpw.at(w_off[i]), 3, 1, 1,
                             dO.at(o_off[i]), int(Ms[i]), dpb.at(b_off[i]), ug.ACT_LEAKY_RELU, 0.1)
              for i in range(count)]
-    plan = ug.GroupedPlan(convs, capture=True)
-    tuned = plan.tune()
-    warm(lambda: plan.run(stream), stream)
     e0, e1 = hip.Event(), hip.Event()
-    reps = 10
-    e0.record(stream)
-    for _ in range(reps):
-        plan.run(stream)
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_ms(e1) / reps
-    # Check the smallest problems against a direct convolution (numpy, FP64).
-    out = dO.download()
+
+    def run_mode(mode):
+        plan = ug.GroupedPlan(convs, mode=mode, capture=True)
+        tuned = plan.tune()
+        warm(lambda: plan.run(stream), stream)
+        reps = 10
+        e0.record(stream)
+        for _ in range(reps):
+            plan.run(stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        plan.destroy()
+        return ug.grouped_variants(mode)[tuned], ms, check()
+
+    def check():
+        return check_convs(dO.download(), px, pw, pb, Cin, HW, Ms, x_off, w_off, b_off, o_off)
+
+    tuned_name, ms, worst = run_mode(ug.MODE_FP32)
+    m32_name, ms_m32, worst_m32 = run_mode(ug.MODE_FP32_MFMA)
+    tf = flop / (ms * 1e-3) / 1e12
+    tf_m32 = flop / (ms_m32 * 1e-3) / 1e12
+    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": tuned_name,
+         "grouped_graph_ms": round(ms, 4), "grouped_tflops": round(tf, 3),
+         "vs_grouped_gemm_same_shapes": round(tf / gemm_grouped_tf, 4) if gemm_grouped_tf else None,
+         "checked_problems_worst_err_over_bound": worst,
+         "fp32_mfma": {"tuned_variant": m32_name, "grouped_graph_ms": round(ms_m32, 4),
+                       "grouped_tflops": round(tf_m32, 3),
+                       "vs_grouped_gemm_same_shapes": round(tf_m32 / m32_grouped_tf, 4) if m32_grouped_tf else None,
+                       "checked_problems_worst_err_over_bound": worst_m32},
+         "note": "3x3 pad-1 convolutions + bias + LeakyReLU(0.1), implicit im2col; GEMM shapes of config 4"}
+    log(f"[config4-conv] grouped implicit conv {tf:.1f} TFLOP/s ({ms:.3f} ms); on FP32 MFMA {tf_m32:.1f} TFLOP/s")
+    for d in (dpx, dpw, dpb, dO):
+        d.free()
+    e0.destroy()
+    e1.destroy()
+    return r
+
+
+def check_convs(out, px, pw, pb, Cin, HW, Ms, x_off, w_off, b_off, o_off):
+    """Worst error over the plain FP32 bound of the four smallest convolutions
+    against a direct FP64 convolution (numpy)."""
     worst = 0.0
-    for i in np.argsort(Ms * HW * HW * Ks)[:4]:
+    for i in np.argsort(Ms * HW * HW * Cin)[:4]:
         c, hw, m = int(Cin[i]), int(HW[i]), int(Ms[i])
         x = px[x_off[i]:x_off[i] + c * hw * hw].reshape(c, hw, hw).astype(np.float64)
         wk = pw[w_off[i]:w_off[i] + m * c * 9].reshape(m, c, 3, 3).astype(np.float64)
@@ -388,19 +418,7 @@ def config4_conv(stream, gemm_grouped_tf):
         sc = sc.reshape(m, -1) + np.abs(b)[:, None]
         got = out[o_off[i]:o_off[i] + m * hw * hw].reshape(m, -1)
         worst = max(worst, float(np.max(np.abs(got - ref) / (2 * (c * 9 + 1) * 2.0 ** -24 * sc))))
-    plan.destroy()
-    tf = flop / (ms * 1e-3) / 1e12
-    r = {"problems": count, "gflop": round(flop / 1e9, 2), "tuned_variant": ug.grouped_variants()[tuned],
-         "grouped_graph_ms": round(ms, 4), "grouped_tflops": round(tf, 3),
-         "vs_grouped_gemm_same_shapes": round(tf / gemm_grouped_tf, 4) if gemm_grouped_tf else None,
-         "checked_problems_worst_err_over_bound": worst,
-         "note": "3x3 pad-1 convolutions + bias + LeakyReLU(0.1), implicit im2col; GEMM shapes of config 4"}
-    log(f"[config4-conv] grouped implicit conv {tf:.1f} TFLOP/s ({ms:.3f} ms)")
-    for d in (dpx, dpw, dpb, dO):
-        d.free()
-    e0.destroy()
-    e1.destroy()
-    return r
+    return worst
 
 
 def blas12(stream):
@@ -743,7 +761,9 @@ def main():
         line["config3"] = config3(stream, args.steps, args.warmup)
         fp32_tf = res.get("fp32", {}).get("tflops")
         line["config4_batched"] = config4(stream, fp32_tf, {k: v.get("tflops") for k, v in res.items()})
-        line["config4_as_convolutions"] = config4_conv(stream, line["config4_batched"].get("grouped_tflops"))
+        line["config4_as_convolutions"] = config4_conv(
+            stream, line["config4_batched"].get("grouped_tflops"),
+            line["config4_batched"].get("fp32_mfma_grouped", {}).get("grouped_tflops"))
         line["split_k"] = split_k(stream)
         line["blas12"] = blas12(stream)
         line["host_pointer_path"] = host_path()

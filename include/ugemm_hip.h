@@ -170,6 +170,12 @@ typedef struct {
 } ugemm_conv_problem_t;
 int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count,
                                        const ugemm_conv_problem_t *convs, int capture, void *stream);
+/* The same with an arithmetic mode: UGEMM_MODE_FP32 (as above) or
+ * UGEMM_MODE_FP32_MFMA (the tiles on the FP32 matrix cores); any other mode
+ * is UGEMM_ERR_UNSUPPORTED. */
+int ugemm_hip_conv_grouped_plan_create_mode(ugemm_grouped_plan_t **plan, int count,
+                                            const ugemm_conv_problem_t *convs, int mode, int capture,
+                                            void *stream);
 /* One-shot convenience: plan, run (no capture), destroy. */
 int ugemm_hip_sgemm_grouped(int count, const ugemm_problem_t *problems, int mode,
                             void *stream);

@@ -32,18 +32,18 @@ CONV_SHAPES = [
 ]
 
 
-def _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, act, slope, variant=None):
+def _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, act, slope, variant=None, mode=ug.MODE_FP32):
     hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
     d_x, d_w = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(wts)
     d_b = hip.DeviceArray.from_host(bias) if bias is not None else None
     d_o = hip.DeviceArray.from_host(np.full((ch, hcol * wcol), np.nan, np.float32))
     if variant is not None:
-        ug.force_variant(ug.MODE_FP32, variant)
+        ug.force_variant(mode, variant)
     try:
-        ug.conv2d_dev(d_x, ich, w, h, d_w, k, pad, stride, d_o, ch, bias=d_b, act=act, slope=slope)
+        ug.conv2d_dev(d_x, ich, w, h, d_w, k, pad, stride, d_o, ch, bias=d_b, act=act, slope=slope, mode=mode)
     finally:
         if variant is not None:
-            ug.force_variant(ug.MODE_FP32, -1)
+            ug.force_variant(mode, -1)
     hip.synchronize()
     return d_o.download()
 
@@ -102,7 +102,37 @@ def test_conv2d_argument_errors():
     assert L.ugemm_hip_conv2d_dev(d.ptr, 1, 4, 4, d.ptr, 3, 0, 1, d.ptr, 1, None, 1, 0.1, 42, None) == ug.ERR_ARG
 
 
-@pytest.mark.parametrize("mode", [ug.MODE_FP32, ug.MODE_BF16X3])
+@pytest.mark.parametrize("shape", CONV_SHAPES)
+@pytest.mark.parametrize("variant", [None, 0, 1, 3])
+def test_conv2d_fp32_mfma(shape, variant):
+    """Convolutions on the FP32 matrix cores: the im2col operand gathered in
+    the FP32-MFMA tile loader, bias and LeakyReLU in its store; plain FP32
+    bound.  Weights with K = ich*k*k not a multiple of 4 take scalar loads."""
+    ich, h, w, k, pad, stride, ch = shape
+    g = orc.rng(zlib.crc32(f"m32conv{shape}".encode()))
+    x = orc.uniform(g, (ich, h, w))
+    wts = orc.uniform(g, (ch, ich * k * k))
+    bias = orc.uniform(g, (ch,))
+    got = _conv_dev(x, wts, bias, ich, h, w, k, pad, stride, ch, ug.ACT_LEAKY_RELU, 0.1, variant=variant,
+                    mode=ug.MODE_FP32_MFMA)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=1, slope=0.1)
+    ok, worst = orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))
+    assert ok, f"{shape} variant {variant}: worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_conv2d_fp32_mfma_every_variant(variant):
+    ich, h, w, k, pad, stride, ch = 24, 20, 21, 3, 1, 1, 70
+    g = orc.rng(77 + variant)
+    x, wts = orc.uniform(g, (ich, h, w)), orc.uniform(g, (ch, ich * k * k))
+    got = _conv_dev(x, wts, None, ich, h, w, k, pad, stride, ch, ug.ACT_NONE, 0.1, variant=variant,
+                    mode=ug.MODE_FP32_MFMA)
+    exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, None, act=0, slope=0.1)
+    ok, worst = orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))
+    assert ok, f"variant {variant}: worst/bound {worst:.3g}"
+
+
+@pytest.mark.parametrize("mode", [ug.MODE_FP32, ug.MODE_BF16X3, ug.MODE_FP32_MFMA])
 @pytest.mark.parametrize("splits", [1, 3, 0])
 def test_conv2d_split_k(mode, splits):
     """A convolution with few output tiles and a long K (ich * k * k = 4608)
@@ -240,7 +270,7 @@ def _conv_batch(n, seed):
     return out
 
 
-def _conv_plan(layers, capture):
+def _conv_plan(layers, capture, mode=ug.MODE_FP32):
     keep, probs, outs = [], [], []
     for (ich, h, w, k, pad, stride, ch, x, wts, bias, act) in layers:
         hcol, wcol = orc.conv_out_shape(h, w, k, pad, stride)
@@ -251,7 +281,7 @@ def _conv_plan(layers, capture):
         outs.append(d_o)
         probs.append(ug.ConvProblem(d_x.ptr, ich, w, h, d_w.ptr, k, pad, stride, d_o.ptr, ch,
                                     d_b.ptr if d_b is not None else None, act, 0.1))
-    return ug.GroupedPlan(probs, capture=capture), outs, keep
+    return ug.GroupedPlan(probs, mode=mode, capture=capture), outs, keep
 
 
 @pytest.mark.parametrize("capture", [False, True])
@@ -292,6 +322,34 @@ def test_grouped_convolutions_every_variant(variant):
     plan.destroy()
 
 
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4])
+def test_grouped_convolutions_fp32_mfma(variant):
+    """Convolution plans on the FP32 matrix cores (the persistent kernel's
+    FP32-MFMA tiles with the implicit im2col operand), captured; two replays
+    bitwise identical, every layer within the plain FP32 bound."""
+    mode = ug.MODE_FP32_MFMA
+    layers = _conv_batch(16, 555 + variant)
+    ug.force_variant(mode, variant)
+    try:
+        plan, outs, keep = _conv_plan(layers, True, mode=mode)
+    finally:
+        ug.force_variant(mode, -1)
+    first = None
+    for _ in range(2):
+        plan.run(None)
+        hip.synchronize()
+        now = [o.download() for o in outs]
+        if first is None:
+            first = now
+        else:
+            assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(first, now))
+    for (ich, h, w, k, pad, stride, ch, x, wts, bias, act), got in zip(layers, first):
+        exact, scale = orc.conv_lrelu(x, ich, w, h, wts, k, pad, stride, ch, bias, act=act, slope=0.1)
+        ok, worst = orc.check_bound(got, exact, scale, orc.tolerance(ich * k * k))
+        assert ok, ((ich, h, w, k, pad, stride, ch), variant, worst)
+    plan.destroy()
+
+
 def test_grouped_convolution_errors():
     L = ug.load()
     h = ctypes.c_void_p()
@@ -301,3 +359,7 @@ def test_grouped_convolution_errors():
     bad = ug.ConvProblem(None, 1, 4, 4, 16, 3, 0, 1, 16, 1, None, 1, 0.1)    # NULL input
     arr = (ug.ConvProblem * 1)(bad)
     assert L.ugemm_hip_conv_grouped_plan_create(ctypes.byref(h), 1, arr, 0, None) == ug.ERR_ARG
+    # Modes other than FP32 and FP32-MFMA are refused before anything is read.
+    for mode in (ug.MODE_BF16X3, ug.MODE_FP8_BLOCK, 42):
+        assert L.ugemm_hip_conv_grouped_plan_create_mode(ctypes.byref(h), 1, arr, mode, 0, None) == \
+            ug.ERR_UNSUPPORTED

@@ -163,12 +163,11 @@ def test_bias_leaky_relu_epilogue(splits):
     assert np.all(np.abs(got - want) <= orc.tolerance(K) * scale + 1e-30)
 
 
-def test_refused_where_not_provided():
+def test_conv_argument_checks():
     L = ug.load()
     assert ug.MODES["fp32_mfma"] == MODE
-    # The mode is checked before any buffer is touched or kernel launched.
-    assert L.ugemm_hip_conv2d_dev(None, 1, 4, 4, None, 3, 1, 1, None, 1, None, 0, 0.1, MODE, None) == \
-        ug.ERR_UNSUPPORTED
+    # NULL buffers are refused before any kernel is launched.
+    assert L.ugemm_hip_conv2d_dev(None, 1, 4, 4, None, 3, 1, 1, None, 1, None, 0, 0.1, MODE, None) == ug.ERR_ARG
 
 
 def test_tuner_and_every_variant_named():

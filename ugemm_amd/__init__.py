@@ -80,6 +80,7 @@ def load(path=None):
         "ugemm_hip_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(Problem), i, i, vp]),
         "ugemm_hip_grouped_run": (i, [vp, vp]),
         "ugemm_hip_conv_grouped_plan_create": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(ConvProblem), i, vp]),
+        "ugemm_hip_conv_grouped_plan_create_mode": (i, [ctypes.POINTER(vp), i, ctypes.POINTER(ConvProblem), i, i, vp]),
         "ugemm_hip_grouped_plan_destroy": (None, [vp]),
         "ugemm_hip_sgemm_grouped": (i, [i, ctypes.POINTER(Problem), i, vp]),
         "ugemm_hip_last_error": (i, []),
@@ -287,8 +288,8 @@ class GroupedPlan:
         s = stream.handle if hasattr(stream, "handle") else stream
         if problems and isinstance(problems[0], ConvProblem):
             self._arr = (ConvProblem * len(problems))(*problems)
-            rc = load().ugemm_hip_conv_grouped_plan_create(ctypes.byref(h), len(problems), self._arr,
-                                                          1 if capture else 0, s)
+            rc = load().ugemm_hip_conv_grouped_plan_create_mode(ctypes.byref(h), len(problems), self._arr, mode,
+                                                               1 if capture else 0, s)
         else:
             self._arr = (Problem * len(problems))(*problems)
             rc = load().ugemm_hip_grouped_plan_create(ctypes.byref(h), len(problems), self._arr, mode,

@@ -65,8 +65,10 @@ __global__ void grouped_reset_heads(int *heads) {
 template <class CF, bool CONV>
 __device__ __forceinline__ void grouped_tile(const DevProblem &p, int m0, int n0, float *lds) {
     const bool aligned = (p.layout & 4) != 0;
-    if constexpr (IsM32<CF>::value) {   // FP32-MFMA plans: GEMM problems only
-        static_assert(!CONV, "FP32-MFMA tiles take no implicit im2col operand");
+    if constexpr (IsM32<CF>::value && CONV) {   // FP32-MFMA convolution plans
+        m32_tile<CF, false, false, 4, true>(p.M, p.N, p.K, p.alpha, p.A, p.lda, nullptr, 0, p.beta, p.C, p.ldc, m0,
+                                            n0, lds, aligned, p.epi, p.cd);
+    } else if constexpr (IsM32<CF>::value) {   // FP32-MFMA GEMM plans
         switch (p.layout & 3) {
             case 0:
                 m32_tile<CF, false, false, 4>(p.M, p.N, p.K, p.alpha, p.A, p.lda, p.B, p.ldb, p.beta, p.C, p.ldc, m0,
@@ -183,12 +185,8 @@ struct VariantOps {
 
 template <class CF, class CS = CF>
 VariantOps ops_of() {
-    if constexpr (IsM32<CF>::value)
-        return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
-                          nullptr, CF::NT};
-    else
-        return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
-                          (const void *)grouped_fp32_kernel<CF, true, CS>, CF::NT};
+    return VariantOps{CF::BM, CF::BN, CF::BK, CS::BM, CS::BN, (const void *)grouped_fp32_kernel<CF, false, CS>,
+                      (const void *)grouped_fp32_kernel<CF, true, CS>, CF::NT};
 }
 
 VariantOps variant_ops(int mode, int v) {
@@ -658,8 +656,15 @@ int ugemm_hip_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const 
 
 int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, const ugemm_conv_problem_t *convs,
                                        int capture, void *stream) {
+    return ugemm_hip_conv_grouped_plan_create_mode(plan, count, convs, UGEMM_MODE_FP32, capture, stream);
+}
+
+int ugemm_hip_conv_grouped_plan_create_mode(ugemm_grouped_plan_t **plan, int count, const ugemm_conv_problem_t *convs,
+                                            int mode, int capture, void *stream) {
     if (!plan) return fail(UGEMM_ERR_ARG, "grouped conv: NULL plan pointer");
     *plan = nullptr;
+    if (mode != UGEMM_MODE_FP32 && mode != UGEMM_MODE_FP32_MFMA)
+        return fail(UGEMM_ERR_UNSUPPORTED, "grouped conv: mode %d not provided (FP32 and FP32-MFMA)", mode);
     if (count < 0 || (count > 0 && !convs)) return fail(UGEMM_ERR_ARG, "grouped conv: bad problem list");
     int rc = ensure_device();
     if (rc) return rc;
@@ -687,7 +692,7 @@ int ugemm_hip_conv_grouped_plan_create(ugemm_grouped_plan_t **plan, int count, c
         }
     }
     ugemm_grouped_plan *p = new ugemm_grouped_plan();
-    p->mode = UGEMM_MODE_FP32;
+    p->mode = mode;
     p->conv = true;
     hipError_t e;
     if ((e = hipMalloc(&p->d_ktab, ktab.size() * sizeof(int2))) != hipSuccess ||

@@ -112,6 +112,7 @@ int m32_variant_count();
 const char *m32_variant_name(int v);
 int m32_pick_variant(const Gemm<float> &g);
 int launch_m32(const Gemm<float> &g, int variant, hipStream_t s);
+int launch_conv_m32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s);
 
 int mfma_variant_count(int mode);
 const char *mfma_variant_name(int mode, int v);

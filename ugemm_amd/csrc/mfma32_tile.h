@@ -67,12 +67,16 @@ __device__ __forceinline__ m32_floatx4 m32_frag4(const float *L, int row, int k)
 // A and B point at the operands' origins (k = 0); vec_ok == false (VEC 4
 // only): the operands are not 16-byte aligned, every chunk is read with
 // scalar loads.  lds: CF::kLdsFloats floats, 16-byte aligned.
-template <class CF, bool TA, bool TB, int VEC>
+// CONV: op(B) is the implicit im2col matrix of cd (B and TB unused; the
+// gather of sgemm_gl1.h:166-190): every thread keeps one column n of the B
+// tile, so the pixel's origin is computed once per tile and each element
+// costs one table read and two compares; the B stage is [k][n].
+template <class CF, bool TA, bool TB, int VEC, bool CONV = false>
 __device__ __forceinline__ void m32_tile(const int M, const int N, const int K, const float alpha,
                                          const float *__restrict__ A, const long lda, const float *__restrict__ B,
                                          const long ldb, const float beta, float *__restrict__ C, const long ldc,
                                          const int m0, const int n0, float *lds, const bool vec_ok = true,
-                                         const Epilogue &epi = Epilogue()) {
+                                         const Epilogue &epi = Epilogue(), const ConvDesc &cd = ConvDesc()) {
     constexpr bool KCA = !TA, KCB = TB;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NT = CF::NT, FM = CF::FM, FN = CF::FN;
     constexpr int LDA = CF::template ld<KCA, BM>(), LDB = CF::template ld<KCB, BN>();
@@ -92,22 +96,55 @@ __device__ __forceinline__ void m32_tile(const int M, const int N, const int K, 
     const bool fast = interior && vec_ok;
     const int ktiles = (K + BK - 1) / BK, kfull = K / BK;
 
+    static_assert(!CONV || (!TB && NT % BN == 0 && BN % 64 == 0), "implicit operand layout");
+    constexpr int CONV_N = CONV ? BK * BN / NT : 1;
+    float cvals[CONV_N];
+    long c_base = 0;
+    int c_h0 = 0, c_w0 = 0;
+    bool c_ncol = false;
+    if constexpr (CONV) {
+        const int n = n0 + t % BN;
+        c_ncol = n < N;
+        const int ho = c_ncol ? n / cd.wcol : 0, wo = c_ncol ? n - ho * cd.wcol : 0;
+        c_h0 = ho * cd.stride - cd.pad;
+        c_w0 = wo * cd.stride - cd.pad;
+        c_base = (long)c_h0 * cd.W + c_w0;
+    }
+
     auto load_tile = [&](int kt) {
         const int k0 = kt * BK;
         const float *a0 = TA ? A + (long)k0 * lda + m0 : A + (long)m0 * lda + k0;
-        const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
-        if (fast && kt < kfull) {
-            ra.template load<false>(a0, lda, t, 0, 0);
-            rb.template load<false>(b0, ldb, t, 0, 0);
+        const int mr = M - m0, nr = N - n0, kr = K - k0;
+        if (fast && kt < kfull) ra.template load<false>(a0, lda, t, 0, 0);
+        else ra.template load<true>(a0, lda, t, KCA ? kr : mr, KCA ? mr : kr, vec_ok);
+        if constexpr (CONV) {
+#pragma unroll
+            for (int i = 0; i < CONV_N; ++i) {
+                // k is the same for the 64 lanes of a wave (BN is a multiple
+                // of 64): one scalar load of the table entry per wave.
+                const int k = __builtin_amdgcn_readfirstlane(k0 + t / BN + i * (NT / BN));
+                float v = 0.0f;
+                if (c_ncol && k < K) {
+                    const int2 e = cd.ktab[k];
+                    const int hi = c_h0 + (e.y >> 16), wi = c_w0 + (e.y & 0xffff);
+                    if ((unsigned)hi < (unsigned)cd.H && (unsigned)wi < (unsigned)cd.W) v = cd.in[c_base + e.x];
+                }
+                cvals[i] = v;
+            }
         } else {
-            const int mr = M - m0, nr = N - n0, kr = K - k0;
-            ra.template load<true>(a0, lda, t, KCA ? kr : mr, KCA ? mr : kr, vec_ok);
-            rb.template load<true>(b0, ldb, t, KCB ? kr : nr, KCB ? nr : kr, vec_ok);
+            const float *b0 = TB ? B + (long)n0 * ldb + k0 : B + (long)k0 * ldb + n0;
+            if (fast && kt < kfull) rb.template load<false>(b0, ldb, t, 0, 0);
+            else rb.template load<true>(b0, ldb, t, KCB ? kr : nr, KCB ? nr : kr, vec_ok);
         }
     };
     auto store_tile = [&](int buf) {
         ra.template store<true, LDA>(As + buf * SA, t);
-        rb.template store<true, LDB>(Bs + buf * SB, t);
+        if constexpr (CONV) {
+#pragma unroll
+            for (int i = 0; i < CONV_N; ++i) Bs[buf * SB + (t / BN + i * (NT / BN)) * LDB + t % BN] = cvals[i];
+        } else {
+            rb.template store<true, LDB>(Bs + buf * SB, t);
+        }
     };
 
     m32_floatx4 acc[FM][FN];

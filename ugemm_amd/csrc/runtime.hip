@@ -470,8 +470,7 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
     int rc = conv_shape(ich, w, h, k, pad, stride, ch, &hcol, &wcol);
     if (rc) return rc;
     if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "conv: unknown mode %d", mode);
-    if (mode == UGEMM_MODE_FP8_BLOCK || mode == UGEMM_MODE_FP32_MFMA)
-        return fail(UGEMM_ERR_UNSUPPORTED, "conv: mode %d is not provided", mode);
+    if (mode == UGEMM_MODE_FP8_BLOCK) return fail(UGEMM_ERR_UNSUPPORTED, "conv: the FP8 form is not provided");
     if (!d_in || !d_weights || !d_out) return fail(UGEMM_ERR_ARG, "conv: NULL buffer");
     const int K = ich * k * k;
     void *ws = nullptr;
@@ -487,7 +486,9 @@ static int run_conv(const float *d_in, int ich, int w, int h, const float *d_wei
         g.splits = forced_splits(mode);
         const ConvDesc cd{d_in, ktab, h, w, wcol, stride, pad};
         const int v = forced_variant(mode);
-        rc = mode == UGEMM_MODE_FP32 ? launch_conv_fp32(g, cd, v, s) : launch_mfma_conv(g, cd, mode, v, s);
+        rc = mode == UGEMM_MODE_FP32        ? launch_conv_fp32(g, cd, v, s)
+             : mode == UGEMM_MODE_FP32_MFMA ? launch_conv_m32(g, cd, v, s)
+                                            : launch_mfma_conv(g, cd, mode, v, s);
     }
     const int rc2 = stream_free(ws, s);
     return rc ? rc : rc2;

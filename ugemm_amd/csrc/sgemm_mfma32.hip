@@ -41,6 +41,32 @@ __global__ __launch_bounds__(CF::NT, CF::MINB) void sgemm_mfma32_kernel(
                                   true, epi);
 }
 
+// Implicit-GEMM convolution on the FP32 matrix cores: C (M = output
+// channels x N = output pixels) = W (M x K, row-major weights) * im2col(in),
+// then the epilogue; the column matrix is gathered in the tile loader
+// (ConvDesc).  Split-K: the chunk's rows of the offset table and its weight
+// columns, as in the GEMM kernel.
+template <class CF>
+__global__ __launch_bounds__(CF::NT, CF::MINB) void conv_mfma32_kernel(
+    int M, int N, int K_, const float *__restrict__ A_, long lda, float *__restrict__ C_, long ldc_, int tiles_m,
+    int tiles_n, Epilogue epi, ConvDesc cd, bool vec, int kc, float *__restrict__ ws) {
+    constexpr int kLds = 2 * (CF::template stage<true, CF::BM>() + CF::template stage<false, CF::BN>());
+    __shared__ __attribute__((aligned(16))) float lds[kLds];
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    if (gridDim.y > 1) {
+        const int k0 = blockIdx.y * kc, kz = min(kc, K_ - k0);
+        ConvDesc cz = cd;
+        cz.ktab = cd.ktab + k0;
+        m32_tile<CF, false, false, 4, true>(M, N, kz, 1.0f, A_ + k0, lda, nullptr, 0, 0.0f,
+                                            ws + (long)blockIdx.y * M * N, (long)N, tm * CF::BM, tn * CF::BN, lds, vec,
+                                            Epilogue(), cz);
+    } else {
+        m32_tile<CF, false, false, 4, true>(M, N, K_, 1.0f, A_, lda, nullptr, 0, 0.0f, C_, ldc_, tm * CF::BM,
+                                            tn * CF::BN, lds, vec, epi, cd);
+    }
+}
+
 using V0 = M32Cfg<128, 128, 16, 2, 2, 2>;
 using V1 = M32Cfg<256, 128, 16, 4, 2>;
 using V2 = M32Cfg<128, 256, 16, 2, 4>;
@@ -92,6 +118,33 @@ int launch_cfg(const Gemm<float> &g, hipStream_t s) {
     return rc;
 }
 
+template <class CF>
+int launch_conv_cfg(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
+    const int tiles_m = (g.M + CF::BM - 1) / CF::BM, tiles_n = (g.N + CF::BN - 1) / CF::BN;
+    const long nblk = (long)tiles_m * tiles_n;
+    if (nblk > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "conv fp32 mfma: too many tiles (%ld)", nblk);
+    const bool vec = ugemm_aligned16(g.A) && (g.lda % 4 == 0);
+    int kc = g.K;
+    const int splits = split_count(g.splits, nblk, g.K, 64, &kc, 2);
+    float *ws = nullptr;
+    if (splits > 1) {
+        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
+        if (rc) return rc;
+    }
+    timing_begin(s);
+    hipLaunchKernelGGL((conv_mfma32_kernel<CF>), dim3((unsigned)nblk, (unsigned)splits), dim3(CF::NT), 0, s, g.M,
+                       g.N, g.K, g.A, g.lda, g.C, g.ldc, tiles_m, tiles_n, g.epi, cd, vec, kc, ws);
+    timing_end(s);
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "conv fp32 mfma launch");
+    if (splits > 1) {
+        if (!rc) rc = launch_splitk_reduce_f32(g.M, g.N, splits, 1.0f, ws, 0.0f, g.C, g.ldc, g.epi, s);
+        const int rc2 = stream_free(ws, s);
+        if (!rc) rc = rc2;
+    }
+    return rc;
+}
+
 }  // namespace
 
 int m32_variant_count() { return kNum; }
@@ -107,6 +160,20 @@ int m32_pick_variant(const Gemm<float> &g) {
     if (t256 >= cus) return 1;
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     return t128 >= cus ? 0 : 3;
+}
+
+// g: the convolution as a GEMM (alpha 1, beta 0, A the weights, B unused).
+int launch_conv_m32(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStream_t s) {
+    if (variant < 0) variant = m32_pick_variant(g);
+    switch (variant) {
+        case 0: return launch_conv_cfg<V0>(g, cd, s);
+        case 1: return launch_conv_cfg<V1>(g, cd, s);
+        case 2: return launch_conv_cfg<V2>(g, cd, s);
+        case 3: return launch_conv_cfg<V3>(g, cd, s);
+        case 4: return launch_conv_cfg<V4>(g, cd, s);
+        case 5: return launch_conv_cfg<V5>(g, cd, s);
+        default: return fail(UGEMM_ERR_ARG, "conv fp32 mfma: no variant %d", variant);
+    }
 }
 
 int launch_m32(const Gemm<float> &g, int variant, hipStream_t s) {
