@@ -71,6 +71,9 @@ def wide(g, shape):
     return (s * np.ldexp(1.0 + u, e)).astype(np.float32)
 
 
+LAST_STEPS = {}   # per-step median and minimum of the last timed() call
+
+
 def timed(fn, steps, warmup, stream, warm_ms=0.0):
     """(ms per step from events, ms per core kernel launch, wall seconds).
 
@@ -89,21 +92,25 @@ def timed(fn, steps, warmup, stream, warm_ms=0.0):
             fn()
         stream.synchronize()
     hip.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
+    # One event between consecutive steps: the first-to-last span gives the
+    # mean; the per-step spans give the median and the minimum.
+    ev = [hip.Event() for _ in range(steps + 1)]
     ug.timing_enable(True)
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
+    ev[0].record(stream)
+    for i in range(steps):
         fn()
-    e1.record(stream)
-    e1.synchronize()
+        ev[i + 1].record(stream)
+    ev[-1].synchronize()
     hip.synchronize()
     wall = time.perf_counter() - t0
     n, core_ms = ug.timing_collect()
     ug.timing_enable(False)
-    ms = e0.elapsed_ms(e1) / steps
-    e0.destroy()
-    e1.destroy()
+    ms = ev[0].elapsed_ms(ev[-1]) / steps
+    per = sorted(ev[i].elapsed_ms(ev[i + 1]) for i in range(steps))
+    LAST_STEPS["median_ms"], LAST_STEPS["min_ms"] = per[len(per) // 2], per[0]
+    for e in ev:
+        e.destroy()
     return ms, (core_ms / n if n else None), wall
 
 
@@ -154,7 +161,8 @@ def sgemm_forms(n, A, B, C0, alpha, beta, forms, steps, warmup, stream, rows, ta
         got = dC.download()[rows]
         peak_key, mult = FORM_PEAK[name]
         tf = flop / (ms * 1e-3) / 1e12
-        rec = {"tflops": round(tf, 3), "ms_per_call": round(ms, 4), "wall_s": round(wall, 3),
+        rec = {"tflops": round(tf, 3), "ms_per_call": round(ms, 4), "ms_median": round(LAST_STEPS["median_ms"], 4),
+               "ms_min": round(LAST_STEPS["min_ms"], 4), "wall_s": round(wall, 3),
                "core_kernel": CORE_KERNEL[name], "core_ms": round(core_ms, 4) if core_ms else None,
                "core_tflops": round(flop / (core_ms * 1e-3) / 1e12, 3) if core_ms else None,
                "frac_of_fp32_vector_peak": round(tf / PEAK["fp32_vector"], 4),
@@ -550,6 +558,41 @@ def host_path():
     return res
 
 
+def config1_cpu():
+    """Config 1: the reference's CPU paths at 1024^3 (row-major NN, alpha 1,
+    beta 0, U[-1,1) from seed 0x55474D4D01, C zero), one thread each, checked
+    elementwise against an FP64 product: sgemm_avx (sgemm_avx256.h:392) and
+    sgemm_c (gemm_cpu.h:284), compiled from the reference sources."""
+    try:
+        from oracle import oracle as orc
+    except Exception as e:  # noqa: BLE001
+        return {"error": f"oracle unavailable: {e}"}
+    L = orc.ref_lib()
+    if L is None:
+        return {"error": "reference library not built"}
+    n = 1024
+    g = np.random.Generator(np.random.PCG64(0x55474D4D01))
+    A, B = uniform(g, (n, n)), uniform(g, (n, n))
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    res = {}
+    for name, cite in (("sgemm_avx", "sgemm_avx256.h:392"), ("sgemm_c", "gemm_cpu.h:284")):
+        fn = getattr(L, name)
+        C = np.zeros((n, n), np.float32)
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn(b"R", b"N", b"N", n, n, n, 1.0, A.ctypes.data, n, B.ctypes.data, n, 0.0, C.ctypes.data, n)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        r = {"source": cite, "threads": 1, "best_s": round(best, 4), "gflops": round(2.0 * n ** 3 / best / 1e9, 2),
+             "max_err_over_bound": round(float(np.max(np.abs(C - exact) / (2 * (n + 1) * 2.0 ** -24 * scale))), 4)}
+        r.update(err_stats(C, exact, scale))
+        res[name] = r
+        log(f"[config1] {name} 1024^3: {r['gflops']:.1f} GFLOP/s, worst/bound {r['max_err_over_bound']}")
+    return res
+
+
 def cpu_baseline(n, seconds_target):
     """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
     by the workload) on a row band of the n^3 workload, one thread."""
@@ -776,6 +819,8 @@ def main():
                                                        max(5, args.steps // 4), 2, stream, rows, "config5")
         del A5, B5, C5
     if not args.no_cpu:
+        if not args.quick:
+            line["config1_cpu"] = config1_cpu()
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 

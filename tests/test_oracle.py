@@ -421,3 +421,35 @@ def test_blas12_oracle_matches_numpy():
         want = 1.5 * a @ x[::incx][:N] + 0.5 * y0[::incy][:M]
         assert np.allclose(y[::incy][:M], want, rtol=1e-13, atol=1e-13)
         assert np.array_equal(y[1::incy], y0[1::incy])          # gaps untouched
+
+
+# ---------------------------------------------------------------------------
+# Config 1: the reference's CPU paths at 1024^3 (row-major NN, alpha 1,
+# beta 0, U[-1,1) from seed 0x55474D4D01, C zero) against FP64.
+
+def test_config1_cpu_paths_at_1024():
+    """sgemm_avx (sgemm_avx256.h:392-455) and sgemm_c (gemm_cpu.h:284-363),
+    compiled from the reference sources, and the oracle's restatement of the
+    gemm_cpu blocking, all within the plain FP32 bound of numpy's FP64
+    product; a planted wrong beta is caught by the same check."""
+    n = 1024
+    g = orc.rng(0x55474D4D01)
+    A, B = orc.uniform(g, (n, n)), orc.uniform(g, (n, n))
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    tol = orc.tolerance(n)
+    runs = {"oracle_sgemm_packed": lambda C: orc.sgemm_packed("R", "N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n)}
+    L = orc.ref_lib()
+    if L is not None:
+        for name in ("sgemm_avx", "sgemm_c"):
+            fn = getattr(L, name)
+            runs[name] = (lambda f: lambda C: f(b"R", b"N", b"N", n, n, n, 1.0, A.ctypes.data, n, B.ctypes.data, n,
+                                                0.0, C.ctypes.data, n))(fn)
+    for name, run in runs.items():
+        C = np.zeros((n, n), np.float32)
+        run(C)
+        ok, worst = orc.check_bound(C, exact, scale, tol)
+        assert ok, f"{name}: worst/bound {worst:.3g}"
+    C = np.ones((n, n), np.float32)          # beta = 0 must not keep C: a stale C is far outside the bound
+    ok, _ = orc.check_bound(C + exact.astype(np.float32), exact, scale, tol)
+    assert not ok
