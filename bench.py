@@ -593,7 +593,7 @@ def config1_cpu():
     return res
 
 
-def cpu_baseline(n, seconds_target):
+def cpu_baseline(n, seconds_target, with_config1=False):
     """The reference's sgemm_avx (sgemm_avx256.h:392-455, the AVX path named
     by the workload) on a row band of the n^3 workload, one thread."""
     try:
@@ -629,6 +629,8 @@ def cpu_baseline(n, seconds_target):
     if L is not None:
         res["all_cores"] = cpu_all_cores(L, n, A, B, seconds_target / 2, cpu)
         res["dgemm"] = cpu_dgemm(L, 4096, seconds_target / 2, cpu)
+        if with_config1:
+            res["config1_1024"] = config1_cpu()
     return res
 
 
@@ -819,9 +821,7 @@ def main():
                                                        max(5, args.steps // 4), 2, stream, rows, "config5")
         del A5, B5, C5
     if not args.no_cpu:
-        if not args.quick:
-            line["config1_cpu"] = config1_cpu()
-        line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds, with_config1=not args.quick)
     print(json.dumps(line), flush=True)
 
 
