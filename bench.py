@@ -26,7 +26,8 @@ Also reported (one JSON line in total):
     one 8192^3 uut call);
   * cpu_baseline: the reference's own sgemm_avx (compiled from its sources
     into oracle/_ref) on a bounded row band of the 8192^3 workload, one thread;
-    beside it the same on every host core, and dgemm_avx on one thread.
+    beside it the same on every host core, dgemm_avx on one thread, and
+    (full run) config 1: sgemm_avx and sgemm_c at 1024^3 against FP64.
 """
 import argparse
 import json
