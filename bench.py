@@ -741,7 +741,11 @@ def main():
                       headline=args.headline)
     del A, B, C0
 
-    head = args.headline if "tflops" in res.get(args.headline, {}) else "fp32"
+    timed_forms = [k for k, v in res.items() if "tflops" in v]
+    if not timed_forms:
+        print("bench.py: no form ran", file=sys.stderr)
+        sys.exit(1)
+    head = args.headline if args.headline in timed_forms else ("fp32" if "fp32" in timed_forms else timed_forms[0])
     h = res[head]
     peak_key, mult = FORM_PEAK[head]
     flop = 2.0 * n ** 3

@@ -174,6 +174,58 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
         prep_kcont_chunk<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, f);
 }
 
+// FP16X3, k-contiguous source with k_pad <= 8192: the row maximum and the
+// split in one pass.  Block r reads row r once into registers (at most 4
+// chunks of 8 per thread), reduces its largest magnitude (amax[r], as the
+// separate row-maximum pass computes it: NaN ignored, 0 for padding rows),
+// then scales by 2^-row_exp and writes the planes -- the operand is read
+// once instead of twice.
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_kcont_rowscaled(const float *__restrict__ X, long ld, int rows,
+                                                                  int K, bool vec, typename FMT::E *__restrict__ out,
+                                                                  long plane_stride, int rows_pad, int k_pad,
+                                                                  unsigned *__restrict__ amax) {
+    static_assert(FMT::SCALED, "row scales");
+    constexpr int MAXC = 4;   // chunks per thread: k_pad <= 4 * 256 * 8
+    const int r = blockIdx.x, t = threadIdx.x;
+    const int nchunk = k_pad / 8;
+    float x[MAXC][8];
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        const int k = (t + i * 256) * 8;
+        const float *p = X + (long)r * ld + k;
+        if (t + i * 256 < nchunk && r < rows && k + 8 <= K && vec) {
+            const float4 a = *reinterpret_cast<const float4 *>(p);
+            const float4 c = *reinterpret_cast<const float4 *>(p + 4);
+            x[i][0] = a.x; x[i][1] = a.y; x[i][2] = a.z; x[i][3] = a.w;
+            x[i][4] = c.x; x[i][5] = c.y; x[i][6] = c.z; x[i][7] = c.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[i][j] = (t + i * 256 < nchunk && r < rows && k + j < K) ? p[j] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m = fmaxf(m, fabsf(x[i][j]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float wmax[4];
+    if ((t & 63) == 0) wmax[t >> 6] = m;
+    __syncthreads();
+    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    const unsigned bits = m > 0.0f ? __float_as_uint(m) : 0u;
+    if (t == 0) amax[r] = bits;
+    const int e = -row_exp(bits);
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        const int c = t + i * 256;
+        if (c >= nchunk) break;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[i][j] = ldexpf(x[i][j], e);
+        write_planes<NP, FMT>(out + blocked_offset(r, c * 8, rows_pad), plane_stride, x[i]);
+    }
+}
+
 // Row-contiguous source (X[k*ld + r]): the 64 x 64 tile (rows r0.., k k0..)
 // transposed through LDS (tile[64][65], 256 threads).
 template <int NP, class FMT>
@@ -942,6 +994,13 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
                  int rows_pad, int k_pad, unsigned *amax, hipStream_t s, bool split = true) {
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
     if constexpr (FMT::SCALED) {
+        if (split && kcont && rows > 0 && K > 0 && k_pad <= 8192) {
+            // Row maxima and planes in one pass (one block per padded row).
+            hipLaunchKernelGGL((split_prep_kcont_rowscaled<NP, FMT>), dim3((unsigned)rows_pad), dim3(256), 0, s, X,
+                               ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax);
+            UGEMM_HIP_TRY(hipGetLastError());
+            return UGEMM_OK;
+        }
         if (rows > 0 && K > 0) {
             if (kcont) {
                 const long kchunks = (K + 2047) / 2048;
