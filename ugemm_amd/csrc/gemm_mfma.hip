@@ -347,7 +347,8 @@ __global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ 
 }
 
 // Row-contiguous source (X[k*ld + r]): block b covers columns r in
-// [256 * (b % cblocks), +256) and k in [256 * (b / cblocks), +256),
+// [256 * (b % cblocks), +256) and k in [S * (b / cblocks), +S), S =
+// kAmaxRcontSpan,
 // cblocks = ceil(rows / 256): consecutive blocks read neighbouring 1 KB row
 // segments (blocks that shared columns and differed only in k read addresses
 // a power-of-two multiple of the row pitch apart, which crowd the same HBM
@@ -355,17 +356,18 @@ __global__ __launch_bounds__(256) void row_amax_kcont(const float *__restrict__ 
 __device__ __forceinline__ void row_amax_rcont_block(const float *__restrict__ X, long ld, int rows, int K, bool vec,
                                                      long kblocks, unsigned *__restrict__ amax, long b) {
     const long cblocks = (rows + 255) / 256;
-    const int c0 = (int)(b % cblocks) * 256, k0 = (int)(b / cblocks) * 256;
+    constexpr int SPAN = kAmaxRcontSpan;
+    const int c0 = (int)(b % cblocks) * 256, k0 = (int)(b / cblocks) * SPAN;
     (void)kblocks;
     const int cg = threadIdx.x & 63, kp = threadIdx.x >> 6;
     const int c = c0 + cg * 4;
     float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (vec && c + 4 <= rows && k0 + 256 <= K) {
-        // Whole block: 64 independent 16-byte loads per thread, issued 16 at
+    if (vec && c + 4 <= rows && k0 + SPAN <= K) {
+        // Whole block: SPAN/4 independent 16-byte loads per thread, issued 16 at
         // a time into registers before any is used (the compiler otherwise
         // waits after every second load, which leaves HBM idle).
         const float *p = X + (long)(k0 + kp) * ld + c;
-        for (int kb = 0; kb < 64; kb += 16) {
+        for (int kb = 0; kb < SPAN / 4; kb += 16) {
             float4 v[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) v[i] = *reinterpret_cast<const float4 *>(p + (long)(kb + i) * 4 * ld);
@@ -376,7 +378,7 @@ __device__ __forceinline__ void row_amax_rcont_block(const float *__restrict__ X
             }
         }
     } else {
-        for (int kk = kp; kk < 256 && k0 + kk < K; kk += 4) {
+        for (int kk = kp; kk < SPAN && k0 + kk < K; kk += 4) {
             const float *p = X + (long)(k0 + kk) * ld + c;
             if (vec && c + 4 <= rows) {
                 const float4 a = *reinterpret_cast<const float4 *>(p);
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ 
 
 // Row maxima of the operands of a grouped FP16X3 plan: job j (PrepJob with
 // out = its maxima, nbx = K chunks per row (k-contiguous: 2048 wide) or per
-// 256-column block (row-contiguous: 256 deep)), blocks as in the kernels above.
+// 256-column block (row-contiguous: kAmaxRcontSpan deep)), blocks as in the kernels above.
 __global__ __launch_bounds__(256) void grouped_row_amax(const PrepJob *__restrict__ jobs, int njobs) {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
@@ -1007,7 +1009,7 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
                 hipLaunchKernelGGL(row_amax_kcont, dim3((unsigned)(rows * kchunks)), dim3(256), 0, s, X, ld, rows, K,
                                    vec, kchunks, amax);
             } else {
-                const long kblocks = (K + 255) / 256, cblocks = (rows + 255) / 256;
+                const long kblocks = (K + kAmaxRcontSpan - 1) / kAmaxRcontSpan, cblocks = (rows + 255) / 256;
                 hipLaunchKernelGGL(row_amax_rcont, dim3((unsigned)(cblocks * kblocks)), dim3(256), 0, s, X, ld, rows,
                                    K, vec, kblocks, amax);
             }

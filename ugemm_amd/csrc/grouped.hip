@@ -419,7 +419,7 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         j.kcont = kcont ? 1 : 0;
         j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
         j.block0 = (int)ablocks;
-        j.nbx = kcont ? (K + 2047) / 2048 : (K + 255) / 256;
+        j.nbx = kcont ? (K + 2047) / 2048 : (K + kAmaxRcontSpan - 1) / kAmaxRcontSpan;
         ablocks += kcont ? (long)rows * j.nbx : (long)((rows + 255) / 256) * j.nbx;
         if (K > 0) ajobs.push_back(j);
         else ablocks = j.block0;   // no rows to scan

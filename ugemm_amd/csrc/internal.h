@@ -37,6 +37,10 @@ struct ConvDesc {
 
 // A problem of a grouped split-form plan: its prepared planes (blocked
 // layout, NP planes of Mp x Kp and Np x Kp) and C; and one C tile of it.
+// k rows per block of the row-maximum pass over a row-contiguous operand
+// (blocks cover 256 columns x this many k).
+constexpr int kAmaxRcontSpan = 128;
+
 struct GroupedSplitProblem {
     const void *Ap, *Bp;
     long a_plane, b_plane;
