@@ -174,6 +174,47 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
         prep_kcont_chunk<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, f);
 }
 
+// k-contiguous source, 2-D blocks: block (bx, by) covers rows [64 bx, +64)
+// and k [128 by, +128) (four 32-wide k tiles).  Chunk f = t + 256 i of a
+// thread is row f / 16, k-chunk f % 16: each load instruction of a wave reads
+// four 512-byte row segments, and each store writes four 256-byte runs of
+// the blocked layout (the rows of one k tile are contiguous there) -- where
+// a row-major walk stored sixteen 64-byte pieces per instruction.  All four
+// chunks are loaded before any is split.
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__restrict__ X, long ld, int rows, int K,
+                                                              bool vec, typename FMT::E *__restrict__ out,
+                                                              long plane_stride, int rows_pad, int k_pad,
+                                                              const unsigned *__restrict__ amax) {
+    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 128, t = threadIdx.x;
+    float x[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int f = t + 256 * i, r = r0 + (f >> 4), k = k0 + (f & 15) * 8;
+        const float *p = X + (long)r * ld + k;
+        if (r < rows && k + 8 <= K && vec) {
+            const float4 a = *reinterpret_cast<const float4 *>(p);
+            const float4 b = *reinterpret_cast<const float4 *>(p + 4);
+            x[i][0] = a.x; x[i][1] = a.y; x[i][2] = a.z; x[i][3] = a.w;
+            x[i][4] = b.x; x[i][5] = b.y; x[i][6] = b.z; x[i][7] = b.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[i][j] = (r < rows && k + j < K) ? p[j] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int f = t + 256 * i, r = r0 + (f >> 4), k = k0 + (f & 15) * 8;
+        if (k >= k_pad) continue;
+        if constexpr (FMT::SCALED) {
+            const int e = -row_exp(amax[r]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[i][j] = ldexpf(x[i][j], e);
+        }
+        write_planes<NP, FMT>(out + blocked_offset(r, k, rows_pad), plane_stride, x[i]);
+    }
+}
+
 // FP16X3, k-contiguous source with k_pad <= 8192: the row maximum and the
 // split in one pass.  Block r reads row r once into registers (at most 4
 // chunks of 8 per thread), reduces its largest magnitude (amax[r], as the
@@ -1019,7 +1060,11 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
         UGEMM_HIP_TRY(hipGetLastError());
         return UGEMM_OK;
     }
-    if (kcont) {
+    if (kcont && rows_pad % 64 == 0 && (k_pad + 127) / 128 < 65536) {
+        dim3 grid((unsigned)(rows_pad / 64), (unsigned)((k_pad + 127) / 128));
+        hipLaunchKernelGGL((split_prep_kcont_tiles<NP, FMT>), grid, dim3(256), 0, s, X, ld, rows, K, vec, out,
+                           plane_stride, rows_pad, k_pad, (const unsigned *)amax);
+    } else if (kcont) {
         const long total = (long)rows_pad * (k_pad / 8);
         // One chunk per thread (the grid-stride loop only matters beyond 2^28
         // chunks): a grid capped at 8192 blocks, four chunks per thread, ran
