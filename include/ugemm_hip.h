@@ -121,9 +121,10 @@ int ugemm_hip_sgemm_ex_dev(char major, char transa, char transb, int M, int N, i
  * (sgemm_gl1.h:191-218) and ocl_convolution_LReLU (sgemm_ocl1.h:302-337):
  * host pointers, synchronous, slope 0.1; status in ugemm_hip_last_error().
  * ugemm_hip_conv2d_dev: device pointers, stream-ordered, act/slope/bias
- * selectable (d_bias may be NULL); mode UGEMM_MODE_FP32 (im2col gathered in
- * the FP32 tile loader) or a split form (BF16X1/X3/X6, FP16X3: the im2col
- * operand is gathered into its planes); FP8 is refused (UNSUPPORTED). */
+ * selectable (d_bias may be NULL); mode UGEMM_MODE_FP32 or
+ * UGEMM_MODE_FP32_MFMA (im2col gathered in the tile loader) or a split form
+ * (BF16X1/X3/X6, FP16X3: the im2col operand is gathered into its planes);
+ * FP8 is refused (UNSUPPORTED). */
 void convolution_LReLU_hip(float *inputs, int ich, int w, int h, float *weights, int k, int pad, int stride,
                            float *outputs, int ch, float *bias);
 int ugemm_hip_conv2d_dev(const float *d_in, int ich, int w, int h, const float *d_weights, int k, int pad,
@@ -148,7 +149,8 @@ typedef struct ugemm_grouped_plan ugemm_grouped_plan_t;
 /* Build a plan: validates the problems, builds the tile table and the
  * per-XCD tile queues, uploads them, and (capture != 0) records the launch
  * in a hipGraph so that ugemm_hip_grouped_run replays one graph launch.
- * mode: UGEMM_MODE_FP32, or a split form (UGEMM_MODE_BF16X1/X3/X6,
+ * mode: UGEMM_MODE_FP32, UGEMM_MODE_FP32_MFMA (the same persistent kernel
+ * with FP32 matrix-core tiles), or a split form (UGEMM_MODE_BF16X1/X3/X6,
  * UGEMM_MODE_FP16X3: the plan owns every problem's prepared planes; a run is
  * a preparation launch and a core launch, FP16X3 adding a row-maximum pass);
  * FP8 returns UGEMM_ERR_UNSUPPORTED. */
