@@ -84,3 +84,18 @@ def test_void_entry_records_status():
     L.sgemm_hip(b"R", b"N", b"N", 4, 4, 8, 1.0, buf.ctypes.data, 7, buf.ctypes.data, 4, 0.0, buf.ctypes.data, 4)
     assert L.ugemm_hip_last_error() == ug.ERR_ARG
     assert b"lda" in L.ugemm_hip_last_error_message()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 (or a multi-rank launch) exits
+    with status 2 and says why, before touching any device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+    env = dict(os.environ, WORLD_SIZE="2")
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py")], capture_output=True, text=True,
+                       timeout=120, cwd=root, env=env)
+    assert r.returncode == 2 and "one GPU" in r.stderr
