@@ -1082,10 +1082,19 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
     return UGEMM_OK;
 }
 
+// Split-K count of the core's grid, with *kc the K chunk (whole 32-wide K tiles).
+template <class CF>
+int core_splits(const Gemm<float> &g, int tiles_m, int tiles_n, int Kp, int *kc) {
+    *kc = Kp;
+    return split_count(g.splits, (long)tiles_m * tiles_n, Kp, CF::BK, kc, CF::NT >= 512 ? 1 : 2);
+}
+
+// The core launch (and, split along K, the reduction of the partial products
+// in ws, splits * M * N floats that the caller allocated with the planes).
 template <class CF, int FA, int FB>
 int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plane, int Mp, const typename CF::FMT::E *Bp,
                 long b_plane, int Np, int Kp, int tiles_m, int tiles_n, const unsigned *amax_a,
-                const unsigned *amax_b, hipStream_t s) {
+                const unsigned *amax_b, int splits, int kc, float *ws, hipStream_t s) {
     auto kern = split_nt_core<CF, FA, FB>;
     // Once per kernel instantiation (thread-safe static initialisation).
     static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
@@ -1097,14 +1106,6 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
                                   : (ugemm_aligned16(X) && ld % 4 == 0);
     };
     const int vec_ab = (vec_ok(g.A, g.lda, FA) ? 1 : 0) | (vec_ok(g.B, g.ldb, FB) ? 2 : 0);
-    // Split-K: chunks of whole 32-wide K tiles.
-    int kc = Kp;
-    const int splits = split_count(g.splits, (long)tiles_m * tiles_n, Kp, CF::BK, &kc, CF::NT >= 512 ? 1 : 2);
-    float *ws = nullptr;
-    if (splits > 1) {
-        const int rc = stream_alloc((void **)&ws, (size_t)splits * g.M * g.N * sizeof(float), s);
-        if (rc) return rc;
-    }
     timing_begin(s);
     hipLaunchKernelGGL(kern, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(CF::NT), CF::LDS_BYTES, s,
                        g.M, g.N, Kp, Ap, a_plane, Mp, Bp, b_plane, Np, g.alpha, g.beta, g.C, g.ldc, tiles_m, tiles_n,
@@ -1113,12 +1114,16 @@ int launch_core(const Gemm<float> &g, const typename CF::FMT::E *Ap, long a_plan
     timing_end(s);
     hipError_t e = hipGetLastError();
     int rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "split core launch");
-    if (splits > 1) {
-        if (!rc) rc = launch_splitk_reduce_f32(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, g.epi, s);
-        const int rc2 = stream_free(ws, s);
-        if (!rc) rc = rc2;
-    }
+    if (!rc && splits > 1) rc = launch_splitk_reduce_f32(g.M, g.N, splits, g.alpha, ws, g.beta, g.C, g.ldc, g.epi, s);
     return rc;
+}
+
+// One workspace per call, allocated before the first launch: the planes and
+// row maxima (base_bytes, then 256 bytes of slack), then the split-K partial
+// products at the next 256-byte boundary.
+static size_t splitk_ws_offset(size_t base_bytes) { return (base_bytes + 256 + 255) / 256 * 256; }
+static size_t splitk_ws_bytes(int splits, const Gemm<float> &g) {
+    return splits > 1 ? (size_t)splits * g.M * g.N * sizeof(float) : 0;
 }
 
 // Workspace: NP planes of A (Mp x Kp) and of B (Np x Kp) -- none for an
@@ -1151,12 +1156,16 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
     if (FMT::SCALED && (long)g.M * ((g.K + 2047) / 2048) > 0x7fffffffL)
         return fail(UGEMM_ERR_ARG, "too many row blocks");
+    int kc;
+    const int splits = core_splits<CF>(g, tiles_m, tiles_n, (int)Kp, &kc);
+    const size_t ws_off = splitk_ws_offset(a_bytes + b_bytes + amax_bytes);
     void *ws = nullptr;
-    int rc = stream_alloc(&ws, a_bytes + b_bytes + amax_bytes + 256, s);
+    int rc = stream_alloc(&ws, ws_off + splitk_ws_bytes(splits, g), s);
     if (rc) return rc;
     E *Ap = (E *)ws, *Bp = (E *)((char *)ws + a_bytes);
     unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + a_bytes + b_bytes) : nullptr;
     unsigned *amax_b = FMT::SCALED ? amax_a + Mp : nullptr;
+    float *part = splits > 1 ? (float *)((char *)ws + ws_off) : nullptr;
     if (FMT::SCALED) {
         // A fill kernel rather than a memset: callers may capture this call in a graph.
         hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
@@ -1167,7 +1176,8 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     if (!rc) {
         const unsigned *ma = amax_a, *mb = amax_b;
 #define UGEMM_CORE(FA_, FB_) \
-    launch_core<CF, FA_, FB_>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, s)
+    launch_core<CF, FA_, FB_>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m, tiles_n, ma, mb, splits, \
+                              kc, part, s)
         if constexpr (CF::FUSABLE) {
             if (fa_mode == FUSE_KCONT) rc = UGEMM_CORE(FUSE_KCONT, FUSE_NONE);
             else if (fb_mode == FUSE_KCONT) rc = UGEMM_CORE(FUSE_NONE, FUSE_KCONT);
@@ -1216,12 +1226,16 @@ int run_cfg_conv(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
     if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
     if (FMT::SCALED && (long)g.M * ((g.K + 2047) / 2048) > 0x7fffffffL)
         return fail(UGEMM_ERR_ARG, "too many row blocks");
+    int kc;
+    const int splits = core_splits<CF>(g, tiles_m, tiles_n, (int)Kp, &kc);
+    const size_t ws_off = splitk_ws_offset(a_bytes + b_bytes + amax_bytes);
     void *ws = nullptr;
-    int rc = stream_alloc(&ws, a_bytes + b_bytes + amax_bytes + 256, s);
+    int rc = stream_alloc(&ws, ws_off + splitk_ws_bytes(splits, g), s);
     if (rc) return rc;
     E *Ap = (E *)ws, *Bp = (E *)((char *)ws + a_bytes);
     unsigned *amax_a = FMT::SCALED ? (unsigned *)((char *)ws + a_bytes + b_bytes) : nullptr;
     unsigned *amax_b = FMT::SCALED ? amax_a + Mp : nullptr;
+    float *part = splits > 1 ? (float *)((char *)ws + ws_off) : nullptr;
     if (FMT::SCALED)
         hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
                            s, amax_a, Mp + Np);
@@ -1240,7 +1254,7 @@ int run_cfg_conv(const Gemm<float> &g, const ConvDesc &cd, hipStream_t s) {
     }
     if (!rc)
         rc = launch_core<CF, FUSE_NONE, FUSE_NONE>(g, Ap, a_plane, (int)Mp, Bp, b_plane, (int)Np, (int)Kp, tiles_m,
-                                                   tiles_n, amax_a, amax_b, s);
+                                                   tiles_n, amax_a, amax_b, splits, kc, part, s);
     const int rc2 = stream_free(ws, s);
     return rc ? rc : rc2;
 }
