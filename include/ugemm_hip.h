@@ -36,7 +36,8 @@ typedef enum {
     UGEMM_ERR_NOMEM = -3,       /* device or host allocation failed */
     UGEMM_ERR_NODEV = -4,       /* no usable gfx950 device */
     UGEMM_ERR_UNSUPPORTED = -5, /* mode/layout combination not provided */
-    UGEMM_ERR_TUNE_FILE = -6    /* tuning cache could not be read or written */
+    UGEMM_ERR_TUNE_FILE = -6,   /* tuning cache could not be read or written */
+    UGEMM_ERR_SCRATCH = -7      /* ugemm_hip_sgemm_noalloc_dev: workspace beyond its static scratch */
 } ugemm_status_t;
 
 /* Arithmetic forms of the SGEMM path. */
@@ -96,6 +97,24 @@ int ugemm_hip_sgemm_dev(char major, char transa, char transb, int M, int N, int 
 int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int K,
                         double alpha, const double *dA, int lda, const double *dB, int ldb,
                         double beta, double *dC, int ldc, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): ugemm_hip_sgemm_dev with every workspace (split-form planes,
+ * FP16X3 row maxima, FP8 planes and block exponents, split-K partial
+ * products) taken from one static device array of
+ * UGEMM_HIP_NOALLOC_SCRATCH_BYTES per GPU, held from library load to process
+ * exit.  It calls no hipMalloc/hipFree.  A call whose workspace would exceed
+ * the array returns UGEMM_ERR_SCRATCH before it launches anything.  The
+ * array keeps the last call's contents (what must start at zero is cleared
+ * by the call), and every call on one GPU shares it: calls that overlap on
+ * one GPU (other streams, other host threads) race and must be serialised
+ * by the caller.  The plain FP32 and FP32-MFMA forms need no workspace
+ * unless split along K, so their size is not limited by the array. */
+#define UGEMM_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ugemm_hip_sgemm_noalloc_dev(char major, char transa, char transb, int M, int N, int K,
+                                float alpha, const float *dA, int lda, const float *dB, int ldb,
+                                float beta, float *dC, int ldc, int mode, void *stream);
 
 /* Bias + activation epilogue, applied after alpha/beta in this order:
  *   C[m][n] += bias[m] (UGEMM_BIAS_ROWS) or bias[n] (UGEMM_BIAS_COLS);

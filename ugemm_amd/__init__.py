@@ -25,7 +25,9 @@ MODE_DGEMM = -1  # variant tables of the DGEMM family
 MODES = {"fp32": MODE_FP32, "bf16x3": MODE_BF16X3, "bf16x1": MODE_BF16X1,
          "fp8": MODE_FP8_BLOCK, "bf16x6": MODE_BF16X6, "fp16x3": MODE_FP16X3, "fp32_mfma": MODE_FP32_MFMA}
 
-OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_UNSUPPORTED, ERR_TUNE_FILE = 0, -1, -2, -3, -4, -5, -6
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_UNSUPPORTED, ERR_TUNE_FILE, ERR_SCRATCH = 0, -1, -2, -3, -4, -5, -6, -7
+# Size of the static device scratch of ugemm_hip_sgemm_noalloc_dev (UGEMM_HIP_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _lib = None
 
@@ -74,6 +76,7 @@ def load(path=None):
         "sgemm_hip_packed": (None, [c, c, i, i, i, f, vp, vp, f, vp]),
         "ugemm_hip_sgemm_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp]),
         "ugemm_hip_dgemm_dev": (i, [c, c, c, i, i, i, d, vp, i, vp, i, d, vp, i, vp]),
+        "ugemm_hip_sgemm_noalloc_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp]),
         "ugemm_hip_sgemm_ex_dev": (i, [c, c, c, i, i, i, f, vp, i, vp, i, f, vp, i, i, vp, i, i, f, vp]),
         "ugemm_hip_conv2d_dev": (i, [vp, i, i, i, vp, i, i, i, vp, i, vp, i, f, i, vp]),
         "convolution_LReLU_hip": (None, [vp, i, i, i, vp, i, i, i, vp, i, vp]),
@@ -170,6 +173,17 @@ def sgemm_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC,
     s = stream.handle if hasattr(stream, "handle") else stream
     rc = load().ugemm_hip_sgemm_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
                                     _ptr(dB), ldb, beta, _ptr(dC), ldc, mode, s)
+    return _raise_if(rc)
+
+
+def sgemm_noalloc_dev(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, mode=MODE_FP32,
+                      stream=None):
+    """sgemm_dev whose workspace comes from the library's static device scratch
+    (no device allocation); raises UgemmError(ERR_SCRATCH) for a call whose
+    workspace exceeds NOALLOC_SCRATCH_BYTES."""
+    s = stream.handle if hasattr(stream, "handle") else stream
+    rc = load().ugemm_hip_sgemm_noalloc_dev(_b(major), _b(transa), _b(transb), M, N, K, alpha, _ptr(dA), lda,
+                                            _ptr(dB), ldb, beta, _ptr(dC), ldc, mode, s)
     return _raise_if(rc)
 
 

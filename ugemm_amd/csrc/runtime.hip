@@ -44,6 +44,27 @@ static int ok() {
 }
 
 // ---------------------------------------------------------------------------
+// Static device scratch of ugemm_hip_sgemm_noalloc_dev. Each device holds its
+// own copy from the load of this code object until the process ends. In host
+// code `scratch` names a host-side stand-in, never the device memory: host
+// code uses the device address looked up once per device at initialisation
+// (g_scratch_addr), and kernels receive that address.
+static __device__ __align__(256) unsigned char scratch[UGEMM_HIP_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_addr[kMaxDevices];        // device address of `scratch` per device
+static hipError_t g_scratch_err[kMaxDevices];    // result of the lookup
+
+// Called with `device` current, under the device mutex.
+static void lookup_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices || g_scratch_addr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_addr[device] = p;
+    else (void)hipGetLastError();   // reported by the entry point that needs it
+}
+
+// ---------------------------------------------------------------------------
 // Device state. One device per process image of the library; the host-pointer
 // entry points run on it, the device-pointer ones on the caller's stream.
 struct DeviceState {
@@ -78,6 +99,7 @@ static int init_locked(int device, size_t max_bytes) {
                         prop.gcnArchName);
         g_dev.device = device;
         g_dev.cus = prop.multiProcessorCount;
+        lookup_scratch(device);
         UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
         UGEMM_HIP_TRY(hipStreamCreateWithFlags(&g_dev.kstream, hipStreamNonBlocking));
         // Keep freed workspace blocks in the default pool (no release at
@@ -110,16 +132,35 @@ int ensure_device() {
 
 int device_cu_count() { return g_dev.ready ? g_dev.cus : 256; }
 
+// While a no-allocation call runs on this thread, workspace requests are
+// carved from the scratch in order (one request per call today) and never
+// reach the allocator; releasing is a no-op.
+struct ScratchArena {
+    char *base;
+    size_t cap, used;
+};
+static thread_local ScratchArena *t_arena = nullptr;
+
 int stream_alloc(void **ptr, size_t bytes, hipStream_t s) {
     *ptr = nullptr;
     if (bytes == 0) return UGEMM_OK;
+    if (ScratchArena *a = t_arena) {
+        const size_t off = (a->used + 255) / 256 * 256;
+        if (off > a->cap || bytes > a->cap - off)
+            return fail(UGEMM_ERR_SCRATCH,
+                        "no-allocation call: workspace of %zu bytes exceeds the %zu-byte static scratch", off + bytes,
+                        a->cap);
+        *ptr = a->base + off;
+        a->used = off + bytes;
+        return UGEMM_OK;
+    }
     hipError_t e = hipMallocAsync(ptr, bytes, s);
     if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(workspace)");
     return UGEMM_OK;
 }
 
 int stream_free(void *ptr, hipStream_t s) {
-    if (!ptr) return UGEMM_OK;
+    if (!ptr || t_arena) return UGEMM_OK;
     hipError_t e = hipFreeAsync(ptr, s);
     if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(workspace)");
     return UGEMM_OK;
@@ -610,6 +651,7 @@ const char *ugemm_hip_status_string(int status) {
         case UGEMM_ERR_NODEV: return "no usable gfx950 device";
         case UGEMM_ERR_UNSUPPORTED: return "unsupported mode or layout";
         case UGEMM_ERR_TUNE_FILE: return "tuning cache file error";
+        case UGEMM_ERR_SCRATCH: return "workspace exceeds the static scratch of the no-allocation entry point";
         default: return "unknown status";
     }
 }
@@ -679,6 +721,27 @@ int ugemm_hip_sgemm_dev(char major, char transa, char transb, int M, int N, int 
     rc = normalize(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, g);
     if (rc) return rc;
     rc = run_sgemm(g, mode, -1, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+
+int ugemm_hip_sgemm_noalloc_dev(char major, char transa, char transb, int M, int N, int K, float alpha,
+                                const float *dA, int lda, const float *dB, int ldb, float beta, float *dC, int ldc,
+                                int mode, void *stream) {
+    if (mode < 0 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
+    int rc = ensure_device();
+    if (rc) return rc;
+    const int dev = g_dev.device;
+    if (dev >= kMaxDevices) return fail(UGEMM_ERR_NODEV, "no-allocation call: device %d has no scratch slot", dev);
+    if (!g_scratch_addr[dev])
+        return hip_fail(g_scratch_err[dev] != hipSuccess ? g_scratch_err[dev] : hipErrorInvalidSymbol,
+                        "no-allocation call: hipGetSymbolAddress(scratch)");
+    Gemm<float> g;
+    rc = normalize(major, transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, g);
+    if (rc) return rc;
+    ScratchArena arena{(char *)g_scratch_addr[dev], (size_t)UGEMM_HIP_NOALLOC_SCRATCH_BYTES, 0};
+    t_arena = &arena;
+    rc = run_sgemm(g, mode, -1, (hipStream_t)stream);
+    t_arena = nullptr;
     return rc ? rc : ok();
 }
 
