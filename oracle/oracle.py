@@ -340,6 +340,60 @@ def fp16x3_product(Aop, Bop):
 
 
 # ---------------------------------------------------------------------------
+# BF16X1 / BF16X3 / BF16X6: the NP-plane bf16 split of every FP32 operand and
+# the product set the split core issues.  The reference has no such path;
+# this restates the build's own rule so that the forms can be checked element
+# by element.  Pinned (tests/test_oracle.py) by an independent nearest-value
+# search for the rounding, the planes' reconstruction bounds, exact products
+# of small integers, and a planted truncating split that must fail.
+
+def bf16_round(x):
+    """float32 values -> the nearest bf16 value (8 significant bits, FP32's
+    exponent range), ties to even, computed on the float32 bit pattern;
+    returned as float32.  Infinities and NaN pass unchanged; a finite value
+    that rounds past the largest bf16 becomes infinite."""
+    x = np.asarray(x, np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return np.where(np.isfinite(x), r.view(np.float32), x)
+
+
+def bf16_split(X, NP):
+    """X (float32 values) -> NP float64 planes h0 = bf16(x), h1 = bf16(x - h0),
+    h2 = bf16(x - h0 - h1); the differences are exact in float32.  The low
+    planes of a non-finite x are 0."""
+    X = np.asarray(X, np.float32)
+    h = bf16_round(X)
+    planes = [h.astype(np.float64)]
+    if NP > 1:
+        with np.errstate(invalid="ignore"):
+            r = np.where(np.isfinite(h), X - h, np.float32(0.0)).astype(np.float32)
+        l1 = bf16_round(r)
+        planes.append(l1.astype(np.float64))
+        if NP > 2:
+            planes.append(bf16_round((r - l1).astype(np.float32)).astype(np.float64))
+    return planes
+
+
+def bf16_products(NP):
+    """The (i, j) plane pairs the split core multiplies: i + j <= NP - 1
+    (NP = 1: h0h0; 2: + h0h1, h1h0; 3: + h1h1, h0h2, h2h0)."""
+    return [(i, j) for i in range(NP) for j in range(NP) if i + j <= NP - 1]
+
+
+def bf16_product(Aop, Bop, NP):
+    """The value a BF16 form with NP planes computes for op(A) @ op(B) before
+    alpha/beta, in float64: sum over bf16_products(NP) of A_i @ B_j."""
+    ap = bf16_split(Aop, NP)
+    bp = bf16_split(np.asarray(Bop, np.float32).T, NP)
+    P = None
+    for (i, j) in bf16_products(NP):
+        t = ap[i] @ bp[j].T
+        P = t if P is None else P + t
+    return P
+
+
+# ---------------------------------------------------------------------------
 # Convolution (sgemm_gl1.h:166-218): bindings to oracle.c.
 
 def conv_out_shape(h, w, k, pad, stride):

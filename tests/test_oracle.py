@@ -313,6 +313,120 @@ def test_fp16x3_product_exact_on_small_integers_and_catches_dropped_terms():
 
 
 # ---------------------------------------------------------------------------
+# BF16 split emulation (the BF16X1 / BF16X3 / BF16X6 forms).
+
+def _bf16_nearest(x):
+    """Independent restatement of round-to-nearest-even onto bf16 for one
+    finite float32: the two bf16 neighbours of x are its float32 bit pattern
+    truncated to the top 16 bits and the next bf16 away from zero; the nearer
+    wins, a tie goes to the one whose last kept bit (bit 16) is 0."""
+    u = int(np.float32(x).view(np.uint32))
+    lo = u & 0xFFFF0000
+    hi = lo + 0x10000
+    v_lo = float(np.uint32(lo).view(np.float32))
+    v_hi = float(np.uint32(hi).view(np.float32)) if ((hi >> 23) & 0xFF) != 0xFF else float("inf") * np.sign(x)
+    d_lo, d_hi = abs(float(x) - v_lo), abs(v_hi - float(x))
+    if d_lo < d_hi or (d_lo == d_hi and (lo >> 16) % 2 == 0):
+        return v_lo
+    return v_hi
+
+
+def _trunc_bf16(x):
+    x = np.asarray(x, np.float32)
+    return (x.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)
+
+
+def _trunc_split_product(Aop, Bop, NP):
+    """A planted mistake: the BF16 split with truncation instead of rounding."""
+    def split(X):
+        X = np.asarray(X, np.float32)
+        h = _trunc_bf16(X)
+        out = [h.astype(np.float64)]
+        r = (X - h).astype(np.float32)
+        if NP > 1:
+            l1 = _trunc_bf16(r)
+            out.append(l1.astype(np.float64))
+            if NP > 2:
+                out.append(_trunc_bf16((r - l1).astype(np.float32)).astype(np.float64))
+        return out
+    ap, bp = split(Aop), split(np.asarray(Bop, np.float32).T)
+    return sum(ap[i] @ bp[j].T for (i, j) in orc.bf16_products(NP))
+
+
+def test_bf16_round_matches_nearest_value_search():
+    g = orc.rng(95)
+    bits = g.integers(0, 0x7F7F0000, size=4000, dtype=np.int64).astype(np.uint32)   # every exponent, subnormals too
+    xs = list(bits.view(np.float32)) + list(-bits[:500].view(np.float32))
+    # exact ties (a 1 in bit 15 and zeros below) on both parities of bit 16, and values just off a tie
+    for m in (0x3F800000, 0x3F810000, 0x40490000, 0x00010000, 0x7F000000):
+        xs += [np.uint32(m | 0x8000).view(np.float32), np.uint32(m | 0x8001).view(np.float32),
+               np.uint32(m | 0x7FFF).view(np.float32)]
+    xs = np.array(xs, np.float32)
+    got = orc.bf16_round(xs)
+    want = np.array([_bf16_nearest(x) for x in xs], np.float64)
+    assert np.array_equal(got.astype(np.float64), want)
+    assert np.all((got.view(np.uint32) & 0xFFFF) == 0)
+    assert orc.bf16_round(np.float32(1.0 + 2.0 ** -8)) == 1.0                 # tie to even (down)
+    assert orc.bf16_round(np.float32(1.0 + 3 * 2.0 ** -8)) == 1.0 + 2.0 ** -6  # tie to even (up)
+    sp = orc.bf16_round(np.array([np.inf, -np.inf, np.nan], np.float32))
+    assert sp[0] == np.inf and sp[1] == -np.inf and np.isnan(sp[2])
+
+
+def test_bf16_split_reconstruction_bounds_are_tight():
+    """|x - h0| <= 2^-8|x| and |x - h0 - h1| <= 2^-17|x| (8 significant bits per
+    plane, rounding to nearest: |x - h0| < 2^-8 |x| puts x - h0 at least one
+    binade lower), each reached within a factor 2 on random data; three planes
+    hold a normal float32 exactly (the last residual has at most 8 significant
+    bits); every plane holds bf16 values."""
+    g = orc.rng(96)
+    X = orc.wide_range(g, (64, 256), emin=-30, emax=30)
+    x = X.astype(np.float64)
+    h = orc.bf16_split(X, 3)
+    for p in h:
+        assert np.all((p.astype(np.float32).view(np.uint32) & 0xFFFF) == 0)
+    for n, lim in ((1, 2.0 ** -8), (2, 2.0 ** -17)):
+        res = np.abs(x - sum(h[:n]))
+        assert np.all(res <= lim * np.abs(x))
+        assert np.max(res / np.abs(x)) > lim / 2
+    assert np.array_equal(h[0] + h[1] + h[2], x)
+    assert [len(orc.bf16_split(X, n)) for n in (1, 2, 3)] == [1, 2, 3]
+    assert orc.bf16_products(1) == [(0, 0)]
+    assert sorted(orc.bf16_products(2)) == [(0, 0), (0, 1), (1, 0)]
+    assert sorted(orc.bf16_products(3)) == [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (2, 0)]
+
+
+def test_bf16_product_exact_on_small_integers_and_catches_dropped_terms():
+    g = orc.rng(97)
+    A = g.integers(-128, 129, size=(9, 40)).astype(np.float32)     # 8 significant bits: h0 = x
+    B = g.integers(-128, 129, size=(40, 11)).astype(np.float32)
+    for NP in (1, 2, 3):
+        assert np.array_equal(orc.bf16_product(A, B, NP), A.astype(np.float64) @ B.astype(np.float64))
+    A = orc.uniform(g, (32, 512)); B = orc.uniform(g, (512, 24))
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    # Per-product bounds of the dropped terms: BF16X3 3*2^-16, BF16X6 8*2^-24;
+    # the next smaller product set misses each.
+    assert np.all(np.abs(orc.bf16_product(A, B, 2) - exact) <= 3 * 2.0 ** -16 * scale)
+    assert not np.all(np.abs(orc.bf16_product(A, B, 1) - exact) <= 3 * 2.0 ** -16 * scale)
+    assert np.all(np.abs(orc.bf16_product(A, B, 3) - exact) <= 8 * 2.0 ** -24 * scale)
+    assert not np.all(np.abs(orc.bf16_product(A, B, 2) - exact) <= 8 * 2.0 ** -24 * scale)
+
+
+@pytest.mark.parametrize("NP,K,M", [(1, 5, 64), (1, 64, 64), (2, 5, 64), (2, 24, 64), (3, 1, 256)])
+def test_bf16_emulation_check_rejects_a_truncating_split(NP, K, M):
+    """The GPU comparison (tests/test_gpu_bf16_emulation.py) holds a result to
+    the emulation within the FP32 accumulation bound at these K: a float32
+    rounding of the emulation passes it, a split that truncates fails it."""
+    g = orc.rng(98 + NP * 100 + K)
+    A, B = orc.uniform(g, (M, K)), orc.uniform(g, (K, M))
+    emu = orc.bf16_product(A, B, NP)
+    scale = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    assert orc.check_bound(emu.astype(np.float32), emu, scale, orc.tolerance(K))[0]
+    planted = _trunc_split_product(A, B, NP).astype(np.float32)
+    assert not orc.check_bound(planted, emu, scale, orc.tolerance(K))[0]
+
+
+# ---------------------------------------------------------------------------
 # Convolution oracle (im2col + GEMM + bias + LeakyReLU, sgemm_gl1.h:166-218).
 
 def _direct_conv(x, wts, k, pad, stride, bias, act, slope):
