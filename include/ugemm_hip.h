@@ -232,6 +232,18 @@ const char *ugemm_hip_grouped_mode_variant_name(int mode, int variant);
 /* ------------------------------------------------------------------ */
 void saxpy_hip(int N, float alpha, const float *x, int incx, float *y, int incy);
 void daxpy_hip(int N, double alpha, const double *x, int incx, double *y, int incy);
+/* Three-operand axpy with the argument lists of saxpy_avx / daxpy_avx
+ * (ugemm.h:41-74): z[i] = a * x[i] + y[i] with one rounding (the FMA branch
+ * the reference's -march=native build takes), i < N, z stored with
+ * non-temporal stores.  Unlike the reference, which works in whole vectors
+ * of 8 (4) and so also writes z up to the next multiple and needs 32-byte
+ * aligned arrays, exactly N elements are written, at any alignment; z may
+ * be y.  Host pointers, synchronous, status in ugemm_hip_last_error(); the
+ * _dev versions take device pointers and a stream. */
+void saxpy_avx_hip(int N, float a, float *x, float *y, float *z);
+void daxpy_avx_hip(int N, double a, double *x, double *y, double *z);
+int ugemm_hip_saxpy3_dev(int N, float a, const float *dx, const float *dy, float *dz, void *stream);
+int ugemm_hip_daxpy3_dev(int N, double a, const double *dx, const double *dy, double *dz, void *stream);
 void sgemv_hip(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx,
                float beta, float *y, int incy);
 void dgemv_hip(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
@@ -255,7 +267,18 @@ const char *ugemm_hip_status_string(int status);
  * The tuning replaces nothing in the reference (it has no tuner for this
  * path); the cache file format is documented in tune.hip. */
 int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode);
+/* Time every tile variant of a grouped plan (each run a few times) and
+ * rebuild the plan with the fastest; returns its index.  It first waits for
+ * all work on the device (the caller's uploads of A, B and C), runs on a
+ * private stream, and restores every problem's C (or output) afterwards, so
+ * C is left as the caller left it; it briefly allocates a copy of all C. */
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *plan);
+/* Tuning cache file.  ugemm_hip_tune_load merges a file written by
+ * ugemm_hip_tune_save (entries of another architecture or CU count are
+ * skipped) and returns the number of entries taken.  When the environment
+ * variable UGEMM_HIP_TUNE_CACHE names a file, it is loaded at initialisation
+ * (ugemm_hip_init or the first call), so that uut callers of sgemm_hip pick
+ * up a saved table; a missing or unreadable file is ignored there. */
 int ugemm_hip_tune_load(const char *path);
 int ugemm_hip_tune_save(const char *path);
 /* Variant chosen for a problem (index into the mode's kernel table); -1 = none. */

@@ -392,6 +392,20 @@ void NAME(int N, T alpha, const T *x, int incx, T *y, int incy)                 
 DEFINE_AXPY(float, oracle_saxpy)
 DEFINE_AXPY(double, oracle_daxpy)
 
+/* oracle_saxpy3 / oracle_daxpy3: z[i] = a * x[i] + y[i] with one rounding,
+ * i < N -- the FMA branch of saxpy_avx / daxpy_avx (ugemm.h:41-74), the branch
+ * the reference's -march=native build takes (_mm256_fmadd_ps/pd, ugemm.h:50,
+ * :66).  The reference processes whole vectors of 8 (4) elements, so it also
+ * writes z up to the next multiple; this restatement writes exactly N. */
+void oracle_saxpy3(int N, float a, const float *x, const float *y, float *z)
+{
+	for (int i = 0; i < N; i++) z[i] = fmaf(a, x[i], y[i]);
+}
+void oracle_daxpy3(int N, double a, const double *x, const double *y, double *z)
+{
+	for (int i = 0; i < N; i++) z[i] = fma(a, x[i], y[i]);
+}
+
 #define DEFINE_GEMV(T, NAME)                                                                  \
 void NAME(char trans, int M, int N, T alpha, const T *A, int lda, const T *x, int incx,      \
           T beta, T *y, int incy, int x_by_incy)                                              \

@@ -55,6 +55,10 @@ def lib():
         L.oracle_saxpy.argtypes = [i, f, vp, i, vp, i]
         L.oracle_daxpy.restype = None
         L.oracle_daxpy.argtypes = [i, d, vp, i, vp, i]
+        L.oracle_saxpy3.restype = None
+        L.oracle_saxpy3.argtypes = [i, f, vp, vp, vp]
+        L.oracle_daxpy3.restype = None
+        L.oracle_daxpy3.argtypes = [i, d, vp, vp, vp]
         L.oracle_sgemv.restype = None
         L.oracle_sgemv.argtypes = [c, i, i, f, vp, i, vp, i, f, vp, i, i]
         L.oracle_dgemv.restype = None
@@ -87,6 +91,10 @@ def ref_lib():
         L.saxpy_cpu.argtypes = [i, f, vp, i, vp, i]
         L.daxpy_cpu.restype = None
         L.daxpy_cpu.argtypes = [i, d, vp, i, vp, i]
+        L.saxpy_avx.restype = None
+        L.saxpy_avx.argtypes = [i, f, vp, vp, vp]
+        L.daxpy_avx.restype = None
+        L.daxpy_avx.argtypes = [i, d, vp, vp, vp]
         L.sgemv_cpu.restype = None
         L.sgemv_cpu.argtypes = [c, i, i, f, vp, i, vp, i, f, vp, i]
         L.dgemv_cpu.restype = None
@@ -427,6 +435,16 @@ def conv_lrelu(inputs, ich, w, h, weights, k, pad, stride, ch, bias=None, act=1,
 
 # ---------------------------------------------------------------------------
 # BLAS-1/2 (ugemm.h:76-160): bindings to oracle.c and dense views.
+
+def axpy3(N, a, x, y):
+    """z = a*x + y with one rounding per element (oracle.c: the FMA branch of
+    saxpy_avx / daxpy_avx, ugemm.h:41-74); float32 or float64 by x's dtype."""
+    dt = np.float64 if x.dtype == np.float64 else np.float32
+    x, y = np.ascontiguousarray(x, dt), np.ascontiguousarray(y, dt)
+    z = np.empty(N, dt)
+    (lib().oracle_daxpy3 if dt == np.float64 else lib().oracle_saxpy3)(N, a, _p(x), _p(y), _p(z))
+    return z
+
 
 def axpy(N, alpha, x, incx, y, incy):
     """y updated in place (working precision of y's dtype)."""

@@ -125,3 +125,57 @@ def test_blas12_argument_errors():
     assert L.ugemm_hip_sgemv_dev(b"N", 8, 4, 1.0, d.ptr, 4, d.ptr, 1, 0.0, d.ptr, 1, None) == ug.ERR_ARG   # lda < M
     assert L.ugemm_hip_sgemv_dev(b"T", 4, 8, 1.0, d.ptr, 8, d.ptr, -1, 0.0, d.ptr, 1, None) == ug.ERR_ARG
     assert L.ugemm_hip_saxpy_dev(0, 1.0, None, 1, None, 1, None) == ug.OK
+
+
+# ---------------------------------------------------------------------------
+# Three-operand axpy (saxpy_avx / daxpy_avx, ugemm.h:41-74): z = a*x + y with
+# one rounding, compared bit for bit (both sides are correctly rounded FMAs).
+
+def _axpy3_golden():
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_axpy3.npz"))
+    return d, [ast.literal_eval(t) for t in d["index"]]
+
+
+def test_axpy3_host_entry_matches_reference_golden():
+    """saxpy_avx_hip / daxpy_avx_hip on the golden inputs: the first N outputs
+    equal the reference's bit for bit, and nothing beyond N is written (the
+    reference's whole-vector loop writes up to the next multiple of 8 / 4)."""
+    d, index = _axpy3_golden()
+    for key, name, N, a in index:
+        x, y, zfull = d[key + "_x"], d[key + "_y"], d[key + "_zfull"]
+        z = np.full(len(zfull), 7.0, x.dtype)
+        ug.axpy3(N, a, x, y, z)
+        assert np.array_equal(z[:N].view(np.uint8), zfull[:N].view(np.uint8)), key
+        assert np.all(z[N:] == 7.0), key
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("N,offset", [(1, 0), (5, 1), (1000, 0), (1000, 3), ((1 << 22) + 3, 0), ((1 << 20) + 7, 1)])
+def test_axpy3_device_entry_vs_oracle(dt, N, offset):
+    """Device entry, aligned (vector kernel) and misaligned (scalar kernel)
+    starts, in place (z = y) and out of place, vs the oracle's FMA loop."""
+    g = orc.rng(N + offset + (7 if dt == np.float64 else 0))
+    x, y = orc.uniform(g, (N + offset,), dtype=dt), orc.uniform(g, (N + offset,), dtype=dt)
+    want = orc.axpy3(N, -0.7, x[offset:], y[offset:])
+    dx, dy, dz = hip.DeviceArray.from_host(x), hip.DeviceArray.from_host(y), hip.DeviceArray((N + offset,), dt)
+    try:
+        ug.axpy3_dev(N, -0.7, dx.at(offset), dy.at(offset), dz.at(offset), double=dt == np.float64)
+        hip.synchronize()
+        assert np.array_equal(dz.download()[offset:], want)
+        ug.axpy3_dev(N, -0.7, dx.at(offset), dy.at(offset), dy.at(offset), double=dt == np.float64)   # z = y
+        hip.synchronize()
+        assert np.array_equal(dy.download()[offset:], want)
+    finally:
+        for d in (dx, dy, dz):
+            d.free()
+
+
+def test_axpy3_argument_errors():
+    L = ug.load()
+    x = np.zeros(4, np.float32)
+    L.saxpy_avx_hip(-1, 1.0, x.ctypes.data, x.ctypes.data, x.ctypes.data)
+    assert L.ugemm_hip_last_error() == ug.ERR_ARG
+    L.saxpy_avx_hip(4, 1.0, x.ctypes.data, None, x.ctypes.data)
+    assert L.ugemm_hip_last_error() == ug.ERR_ARG
+    L.saxpy_avx_hip(0, 1.0, None, None, None)
+    assert L.ugemm_hip_last_error() == ug.OK

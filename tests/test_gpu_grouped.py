@@ -110,6 +110,31 @@ def test_grouped_tuning_keeps_results():
     s.destroy()
 
 
+@pytest.mark.parametrize("mode", [ug.MODE_FP32, ug.MODE_BF16X3])
+def test_grouped_tuning_leaves_c_untouched(mode):
+    """ugemm_hip_tune_grouped runs the plan many times; with beta != 0 that
+    would leave C as neither the input nor one run's result, so the tuner
+    restores every C: after tune() C still holds C0, and one run then meets
+    the bound without re-uploading C.  C0 is uploaded on another stream
+    without a wait: the tuner waits for the device first."""
+    cases = [DevCase(c) for c in uneven_cases(12, 21)]
+    s = hip.Stream()
+    for d in cases:
+        hip.check(hip.lib().hipMemcpyAsync(d.dC.ptr, d.case.C0.ctypes.data, d.dC.nbytes, hip.H2D, s.handle),
+                  "async upload")
+    plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
+    plan.tune()
+    for d in cases:
+        assert np.array_equal(d.dC.download(), d.case.C0)
+    plan.run(s)
+    s.synchronize()
+    for d in cases:
+        ok, worst = d.case.judge(d.dC.download(), mode=mode)
+        assert ok, worst
+    plan.destroy()
+    s.destroy()
+
+
 def test_grouped_mixed_tile_shapes():
     """Variant 4 gives each problem its own tile shape (128 x 128 or 64 x 128
     by the padding of its rows): one plan mixing both, every layout, checked

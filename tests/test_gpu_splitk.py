@@ -170,3 +170,36 @@ def test_tuner_times_split_counts_and_persists_them(tmp_path):
     case = Case("R", "N", "N", M, N, K, 1.0, 0.0, seed=3)
     ok, worst = case.judge(run_host(case, ug.MODE_FP32))
     assert ok, worst
+
+
+def test_tune_cache_environment_variable_loaded_at_init(tmp_path):
+    """UGEMM_HIP_TUNE_CACHE: a fresh process that only initialises the library
+    (as a uut caller of sgemm_hip would) sees the saved table's entry; a
+    missing file is ignored."""
+    import subprocess
+    import sys
+    L = ug.load()
+    M, N, K = 300, 200, 500
+    v = L.ugemm_hip_tune_sgemm(b"N", b"N", M, N, K, ug.MODE_FP32)
+    assert v >= 0
+    path = os.path.join(str(tmp_path), "tune.txt")
+    assert L.ugemm_hip_tune_save(path.encode()) == ug.OK
+    # Rewrite that bucket's variant to another valid one, so that the loaded
+    # entry cannot be mistaken for the built-in choice.
+    want = (v + 1) % len(ug.variants(ug.MODE_FP32))
+    lines = open(path).read().split("\n")
+    lg = lambda x: int(np.ceil(np.log2(x)))
+    for i, l in enumerate(lines[1:], 1):
+        f = l.split()
+        if f and [int(t) for t in f[:6]] == [ug.MODE_FP32, 0, 0, lg(M), lg(N), lg(K)]:
+            f[6] = str(want)
+            lines[i] = " ".join(f)
+    open(path, "w").write("\n".join(lines))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import ugemm_amd as ug; L = ug.load(); ug.init(); "
+            f"print(L.ugemm_hip_tune_lookup(b'N', b'N', {M}, {N}, {K}, 0))")
+    for cache, expect in ((path, want), (os.path.join(str(tmp_path), "missing.txt"), -1)):
+        r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120,
+                           env={**os.environ, "UGEMM_HIP_TUNE_CACHE": cache})
+        assert r.returncode == 0, r.stderr
+        assert int(r.stdout.split()[-1]) == expect, (cache, r.stdout)

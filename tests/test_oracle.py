@@ -10,6 +10,7 @@ Checks against
     beta): the acceptance check must reject each of them;
   * the e4m3 definition (known encodings, ties, subnormals) for the FP8 emulation.
 """
+import ast
 import hashlib
 import json
 import os
@@ -517,6 +518,39 @@ def test_blas12_oracle_matches_reference_golden():
             orc.gemv(trans, M, N, alpha, A, lda, x, incx, beta, y2, incy, x_by_incy=False)
             diverging += not np.allclose(y2, yref, rtol=1e-3)
     assert diverging > 0   # the reference's x-indexing (incy) really differs from incx
+
+
+def _axpy3_golden():
+    d = np.load(os.path.join(GOLDEN, "ref_axpy3.npz"))
+    return d, [ast.literal_eval(t) for t in d["index"]]
+
+
+def test_axpy3_oracle_matches_reference_golden_bitwise():
+    """saxpy_avx / daxpy_avx (ugemm.h:41-74) as the reference's FMA build
+    computes them: the oracle's one-rounding z = a*x + y equals every golden
+    output bit for bit on the first N elements; the reference's whole-vector
+    loop also wrote past N (the divergence the GPU entry avoids)."""
+    d, index = _axpy3_golden()
+    assert len(index) == 20
+    for key, name, N, a in index:
+        x, y, zfull = d[key + "_x"], d[key + "_y"], d[key + "_zfull"]
+        z = orc.axpy3(N, a, x[:N], y[:N])
+        assert np.array_equal(z.view(np.uint8), zfull[:N].view(np.uint8)), key
+        if N == 13:
+            assert np.all(zfull[13:] != 0), key     # written beyond N by the reference
+
+
+def test_axpy3_golden_pins_the_fma_branch():
+    """A planted mistake: the non-FMA evaluation (a*x rounded, then + y) differs
+    from the golden outputs on some elements, so the fixture pins one rounding."""
+    d, index = _axpy3_golden()
+    for key, name, N, a in index:
+        if N < 1000 or a != -0.7:     # (1.5 * a 24-bit input is exact in float64)
+            continue
+        x, y, zfull = d[key + "_x"], d[key + "_y"], d[key + "_zfull"]
+        dt = x.dtype.type
+        two_roundings = (dt(a) * x[:N]).astype(x.dtype) + y[:N]
+        assert not np.array_equal(two_roundings, zfull[:N]), key
 
 
 def test_blas12_oracle_matches_numpy():

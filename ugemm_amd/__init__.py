@@ -105,6 +105,10 @@ def load(path=None):
         "ugemm_hip_version": (cs, []),
         "saxpy_hip": (None, [i, f, vp, i, vp, i]),
         "daxpy_hip": (None, [i, d, vp, i, vp, i]),
+        "saxpy_avx_hip": (None, [i, f, vp, vp, vp]),
+        "daxpy_avx_hip": (None, [i, d, vp, vp, vp]),
+        "ugemm_hip_saxpy3_dev": (i, [i, f, vp, vp, vp, vp]),
+        "ugemm_hip_daxpy3_dev": (i, [i, d, vp, vp, vp, vp]),
         "sgemv_hip": (None, [c, i, i, f, vp, i, vp, i, f, vp, i]),
         "dgemv_hip": (None, [c, i, i, d, vp, i, vp, i, d, vp, i]),
         "ugemm_hip_saxpy_dev": (i, [i, f, vp, i, vp, i, vp]),
@@ -229,6 +233,21 @@ def axpy(N, alpha, x, incx, y, incy):
     fn = L.daxpy_hip if y.dtype == np.float64 else L.saxpy_hip
     fn(N, alpha, _ptr(x), incx, _ptr(y), incy)
     return _raise_if(L.ugemm_hip_last_error())
+
+
+def axpy3(N, a, x, y, z):
+    """Host-array saxpy_avx_hip / daxpy_avx_hip (by z's dtype): z = a*x + y."""
+    L = load()
+    fn = L.daxpy_avx_hip if z.dtype == np.float64 else L.saxpy_avx_hip
+    fn(N, a, _ptr(x), _ptr(y), _ptr(z))
+    return _raise_if(L.ugemm_hip_last_error())
+
+
+def axpy3_dev(N, a, dx, dy, dz, double=False, stream=None):
+    s = stream.handle if hasattr(stream, "handle") else stream
+    L = load()
+    fn = L.ugemm_hip_daxpy3_dev if double else L.ugemm_hip_saxpy3_dev
+    return _raise_if(fn(N, a, _ptr(dx), _ptr(dy), _ptr(dz), s))
 
 
 def gemv(trans, M, N, alpha, A, lda, x, incx, beta, y, incy):

@@ -5,6 +5,7 @@
 //
 // Semantics follow saxpy_cpu / daxpy_cpu (ugemm.h:76-101):
 //   y[n*incy] += alpha * x[n*incx],  n < N
+// saxpy_avx / daxpy_avx (ugemm.h:41-74): z[i] = fma(a, x[i], y[i]), i < N
 // and sgemv_cpu / dgemv_cpu (ugemm.h:103-160), whose element mapping is
 //   'N': y[m*incy] = alpha * sum_n A[m + n*lda] * x[n*incx] + beta * y[m*incy]
 //   'T': y[m*incy] = alpha * sum_n A[n + m*lda] * x[n*incx] + beta * y[m*incy]
@@ -53,6 +54,35 @@ __global__ __launch_bounds__(256) void axpy_vec_kernel(long N, T alpha, const T 
     // tail (fewer than W elements), one thread
     if (blockIdx.x == 0 && threadIdx.x == 0)
         for (long i = nv * W; i < N; ++i) y[i] = fma(alpha, x[i], y[i]);
+}
+
+// Three-operand axpy, the FMA branch of saxpy_avx / daxpy_avx
+// (ugemm.h:41-74): z[i] = fma(a, x[i], y[i]), i < N, z written with
+// non-temporal stores (the reference's _mm256_stream_ps/pd).  z may be y.
+template <typename T>
+__global__ __launch_bounds__(256) void axpy3_kernel(long N, T a, const T *__restrict__ x, const T *y, T *z) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride)
+        __builtin_nontemporal_store(fma(a, x[i], y[i]), z + i);
+}
+
+// 16-byte aligned x, y, z: one 16-byte vector per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void axpy3_vec_kernel(long N, T a, const T *__restrict__ x, const T *y, T *z) {
+    constexpr int W = 16 / sizeof(T);
+    typedef T V __attribute__((ext_vector_type(W)));
+    const long nv = N / W;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const V vx = reinterpret_cast<const V *>(x)[i], vy = reinterpret_cast<const V *>(y)[i];
+        V vz;
+#pragma unroll
+        for (int j = 0; j < W; ++j) vz[j] = fma(a, vx[j], vy[j]);
+        __builtin_nontemporal_store(vz, reinterpret_cast<V *>(z) + i);
+    }
+    // tail (fewer than W elements), one thread
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (long i = nv * W; i < N; ++i) z[i] = fma(a, x[i], y[i]);
 }
 
 // gemv 'N': block (bx, by) covers rows [256*bx, +256) and columns n in
@@ -147,6 +177,23 @@ int axpy(int N, T alpha, const T *x, int incx, T *y, int incy, hipStream_t s) {
 }
 
 template <typename T>
+int axpy3(int N, T a, const T *x, const T *y, T *z, hipStream_t s) {
+    if (N < 0) return fail(UGEMM_ERR_ARG, "axpy3: N=%d", N);
+    if (N == 0) return UGEMM_OK;
+    if (!x || !y || !z) return fail(UGEMM_ERR_ARG, "axpy3: NULL vector");
+    if (ugemm_aligned16(x) && ugemm_aligned16(y) && ugemm_aligned16(z)) {
+        const long nv = N / (16 / (long)sizeof(T));
+        const long blocks = std::max<long>(1, std::min<long>((nv + 255) / 256, 1L << 20));
+        hipLaunchKernelGGL(axpy3_vec_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, (long)N, a, x, y, z);
+    } else {
+        const long blocks = std::max<long>(1, std::min<long>((N + 255L) / 256, 8L * device_cu_count()));
+        hipLaunchKernelGGL(axpy3_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, (long)N, a, x, y, z);
+    }
+    UGEMM_HIP_TRY(hipGetLastError());
+    return UGEMM_OK;
+}
+
+template <typename T>
 int gemv(char trans, int M, int N, T alpha, const T *A, int lda, const T *x, int incx, T beta, T *y, int incy,
          hipStream_t s) {
     if (M < 0 || N < 0) return fail(UGEMM_ERR_ARG, "gemv: M=%d N=%d", M, N);
@@ -201,6 +248,12 @@ int launch_saxpy(int N, float alpha, const float *x, int incx, float *y, int inc
 }
 int launch_daxpy(int N, double alpha, const double *x, int incx, double *y, int incy, hipStream_t s) {
     return axpy<double>(N, alpha, x, incx, y, incy, s);
+}
+int launch_saxpy3(int N, float a, const float *x, const float *y, float *z, hipStream_t s) {
+    return axpy3<float>(N, a, x, y, z, s);
+}
+int launch_daxpy3(int N, double a, const double *x, const double *y, double *z, hipStream_t s) {
+    return axpy3<double>(N, a, x, y, z, s);
 }
 int launch_sgemv(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,
                  float *y, int incy, hipStream_t s) {

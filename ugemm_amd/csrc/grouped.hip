@@ -782,12 +782,42 @@ const char *ugemm_hip_grouped_mode_variant_name(int mode, int v) {
 int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     if (!p) return fail(UGEMM_ERR_ARG, "grouped: NULL plan");
     if (p->grid == 0) return p->variant;
+    // The caller's pending work on any stream (uploads of A, B, C) completes
+    // before the timing runs read the operands.
+    UGEMM_HIP_TRY(hipDeviceSynchronize());
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     UGEMM_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     int best = -1, rc = UGEMM_OK;
     float best_ms = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(UGEMM_ERR_HIP, "events");
+    // Every timing run overwrites each problem's C: keep a copy of every C
+    // (packed, M x N) and put it back afterwards, so that tuning leaves C as
+    // the caller left it.
+    std::vector<size_t> c_off(p->probs.size());
+    size_t c_bytes = 0;
+    for (size_t i = 0; i < p->probs.size(); ++i) {
+        c_off[i] = c_bytes;
+        c_bytes += ((size_t)p->probs[i].M * p->probs[i].N * sizeof(float) + 255) / 256 * 256;
+    }
+    char *c_save = nullptr;
+    if (!rc && c_bytes) {
+        const hipError_t e = hipMalloc((void **)&c_save, c_bytes);
+        if (e != hipSuccess) rc = hip_fail(e, "grouped tune: copy of C");
+    }
+    auto copy_c = [&](bool save) {
+        for (size_t i = 0; i < p->probs.size() && !rc; ++i) {
+            const DevProblem &q = p->probs[i];
+            if (q.M == 0 || q.N == 0) continue;
+            const size_t row = (size_t)q.N * sizeof(float), ld = (size_t)q.ldc * sizeof(float);
+            const hipError_t e = save ? hipMemcpy2DAsync(c_save + c_off[i], row, q.C, ld, row, q.M,
+                                                         hipMemcpyDeviceToDevice, s)
+                                      : hipMemcpy2DAsync(q.C, ld, c_save + c_off[i], row, row, q.M,
+                                                         hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) rc = hip_fail(e, "grouped tune: copy of C");
+        }
+    };
+    if (c_save) copy_c(true);
     const bool cap = p->capture;
     p->capture = false;
     const int nv = grouped_variant_count(p->mode);
@@ -810,11 +840,18 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
         }
     }
     p->capture = cap;
+    if (c_save) {
+        const int rc_run = rc;
+        rc = UGEMM_OK;
+        copy_c(false);
+        if (rc_run) rc = rc_run;
+    }
     if (!rc) rc = p->split ? build_split_schedule(p, best) : build_schedule(p, best);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     hipStreamSynchronize(s);
     hipStreamDestroy(s);
+    if (c_save) hipFree(c_save);
     if (rc) return rc;
     tune_record(TUNE_KEY_GROUPED, p->conv, p->mode, (int)p->probs.size(),
                 (int)std::min(p->flops / 1e6, 2e9), 0, best, best_ms / 3);

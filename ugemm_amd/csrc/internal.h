@@ -155,6 +155,8 @@ int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t
 // ---- BLAS-1/2 helpers (blas12.hip): device pointers, stream-ordered ----
 int launch_saxpy(int N, float alpha, const float *x, int incx, float *y, int incy, hipStream_t s);
 int launch_daxpy(int N, double alpha, const double *x, int incx, double *y, int incy, hipStream_t s);
+int launch_saxpy3(int N, float a, const float *x, const float *y, float *z, hipStream_t s);
+int launch_daxpy3(int N, double a, const double *x, const double *y, double *z, hipStream_t s);
 int launch_sgemv(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,
                  float *y, int incy, hipStream_t s);
 int launch_dgemv(char trans, int M, int N, double alpha, const double *A, int lda, const double *x, int incx,
@@ -187,6 +189,9 @@ int tuned_variant(int mode, int ta, int tb, int M, int N, int K, int *splits = n
 int forced_variant(int mode);                                         // -1 if none
 int forced_splits(int mode);                                          // 0 if none
 void tune_record(int mode, int ta, int tb, int M, int N, int K, int variant, float ms, int splits = 0);
+// Merges a tuning cache file into the table (the device must be initialised);
+// the number of entries taken, or a negative status.
+int tune_load_file(const char *path);
 
 // ---- launch timing (runtime.hip) ----
 // When enabled (ugemm_hip_timing_enable), the dominant kernel of every call

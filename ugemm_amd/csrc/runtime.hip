@@ -111,6 +111,10 @@ static int init_locked(int device, size_t max_bytes) {
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
         }
         g_dev.ready = true;
+        // A saved tuning table for uut callers, who cannot call
+        // ugemm_hip_tune_load themselves; a missing file is not an error.
+        const char *tc = getenv("UGEMM_HIP_TUNE_CACHE");
+        if (tc && *tc) (void)tune_load_file(tc);
     }
     if (max_bytes > g_dev.staging_bytes) {
         if (g_dev.staging) {
@@ -585,6 +589,47 @@ static void host_axpy(int N, T alpha, const T *x, int incx, T *y, int incy) {
     if (!rc && !rc2) ok();
 }
 
+// Host-pointer three-operand axpy (the argument list of saxpy_avx /
+// daxpy_avx, ugemm.h:41-74): stage x and y, run, copy z back, wait.
+static int axpy3_dispatch(int N, float a, const float *x, const float *y, float *z, hipStream_t s) {
+    return launch_saxpy3(N, a, x, y, z, s);
+}
+static int axpy3_dispatch(int N, double a, const double *x, const double *y, double *z, hipStream_t s) {
+    return launch_daxpy3(N, a, x, y, z, s);
+}
+template <typename T>
+static void host_axpy3(int N, T a, const T *x, const T *y, T *z) {
+    int rc = ensure_device();
+    if (rc) return;
+    if (N < 0) {
+        fail(UGEMM_ERR_ARG, "axpy3: N=%d", N);
+        return;
+    }
+    if (N == 0) {
+        ok();
+        return;
+    }
+    if (!x || !y || !z) {
+        fail(UGEMM_ERR_ARG, "axpy3: NULL vector");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    hipStream_t s = g_dev.stream;
+    const size_t n = (size_t)N, span = (n * sizeof(T) + 255) / 256 * 256;
+    void *ws = nullptr;
+    if (hipSetDevice(g_dev.device) != hipSuccess || stream_alloc(&ws, 2 * span, s)) return;
+    T *dx = (T *)ws, *dy = (T *)((char *)ws + span);
+    hipError_t e = hipMemcpyAsync(dx, x, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, n * sizeof(T), hipMemcpyHostToDevice, s);
+    // z overwrites y's device copy (the kernel reads y[i] before writing z[i]).
+    rc = e == hipSuccess ? axpy3_dispatch(N, a, dx, dy, dy, s) : hip_fail(e, "axpy3: copy in");
+    if (!rc && (e = hipMemcpyAsync(z, dy, n * sizeof(T), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        rc = hip_fail(e, "axpy3: copy out");
+    const int rc2 = stream_free(ws, s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess && !rc) rc = hip_fail(e, "axpy3: synchronize");
+    if (!rc && !rc2) ok();
+}
+
 template <typename T>
 static void host_gemv(char trans, int M, int N, T alpha, const T *A, int lda, const T *x, int incx, T beta, T *y,
                       int incy) {
@@ -848,6 +893,18 @@ void saxpy_hip(int N, float alpha, const float *x, int incx, float *y, int incy)
 }
 void daxpy_hip(int N, double alpha, const double *x, int incx, double *y, int incy) {
     host_axpy<double>(N, alpha, x, incx, y, incy);
+}
+void saxpy_avx_hip(int N, float a, float *x, float *y, float *z) { host_axpy3<float>(N, a, x, y, z); }
+void daxpy_avx_hip(int N, double a, double *x, double *y, double *z) { host_axpy3<double>(N, a, x, y, z); }
+int ugemm_hip_saxpy3_dev(int N, float a, const float *dx, const float *dy, float *dz, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_saxpy3(N, a, dx, dy, dz, (hipStream_t)stream);
+    return rc ? rc : ok();
+}
+int ugemm_hip_daxpy3_dev(int N, double a, const double *dx, const double *dy, double *dz, void *stream) {
+    int rc = ensure_device();
+    if (!rc) rc = launch_daxpy3(N, a, dx, dy, dz, (hipStream_t)stream);
+    return rc ? rc : ok();
 }
 void sgemv_hip(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,
                float *y, int incy) {

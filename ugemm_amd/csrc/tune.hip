@@ -209,6 +209,14 @@ int ugemm_hip_tune_load(const char *path) {
     if (!path) return fail(UGEMM_ERR_ARG, "tune_load: NULL path");
     int rc = ensure_device();
     if (rc) return rc;
+    return ugemm::tune_load_file(path);
+}
+
+}  // extern "C"
+
+namespace ugemm {
+
+int tune_load_file(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return fail(UGEMM_ERR_TUNE_FILE, "cannot read %s", path);
     char arch[64], farch[64];
@@ -236,4 +244,4 @@ int ugemm_hip_tune_load(const char *path) {
     return loaded;
 }
 
-}  // extern "C"
+}  // namespace ugemm
