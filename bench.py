@@ -856,6 +856,8 @@ def main():
     # form that does (plain FP32 when none does).
     ap.add_argument("--headline", default="fp16x3")
     ap.add_argument("--no-grid", action="store_true", help="skip config 2's 2048/4096/8192 x NN/NT/TN grid")
+    ap.add_argument("--only", default="", help="comma list of the sections to run after the headline forms "
+                    "(grid, config3, config4, conv, splitk, blas12, host, config5); default: all")
     ap.add_argument("--quick", action="store_true", help="headline forms only")
     ap.add_argument("--dgemm", action="store_true", help="with --quick: also the 4096^3 DGEMM")
     ap.add_argument("--no-cpu", action="store_true")
@@ -965,18 +967,26 @@ def main():
         dst.free()
     if args.quick and args.dgemm:
         line["dgemm_4096"] = dgemm_4096(stream, args.steps)
-    if not args.quick and not args.no_grid:
+    only = set(x for x in args.only.split(",") if x)
+    want = (lambda sec: not args.quick and (not only or sec in only))
+    if want("grid") and not args.no_grid:
         line["config2_grid"] = config2_grid(stream, args.steps)
-    if not args.quick:
+    if want("config3"):
         line["config3"] = config3(stream, args.steps, args.warmup)
+    if want("config4"):
         fp32_tf = res.get("fp32", {}).get("tflops")
         line["config4_batched"] = config4(stream, fp32_tf, {k: v.get("tflops") for k, v in res.items()})
-        line["config4_as_convolutions"] = config4_conv(
-            stream, line["config4_batched"].get("grouped_tflops"),
-            line["config4_batched"].get("fp32_mfma_grouped", {}).get("grouped_tflops"))
+    if want("conv"):
+        c4 = line.get("config4_batched", {})
+        line["config4_as_convolutions"] = config4_conv(stream, c4.get("grouped_tflops"),
+                                                       c4.get("fp32_mfma_grouped", {}).get("grouped_tflops"))
+    if want("splitk"):
         line["split_k"] = split_k(stream)
+    if want("blas12"):
         line["blas12"] = blas12(stream)
+    if want("host"):
         line["host_pointer_path"] = host_path()
+    if want("config5"):
         g5 = np.random.Generator(np.random.PCG64(0x55474D4D05))
         A5, B5 = wide(g5, (n, n)), wide(g5, (n, n))
         C5 = np.zeros((n, n), np.float32)

@@ -190,6 +190,69 @@ def test_grouped_split_forms(mode, variant):
     plan.destroy()
 
 
+def uniform_b_cases(n, seed, tb, sizes=(7, 14, 28)):
+    """im2col-shaped problems whose B all have layout tb (A mixed), with
+    padded and odd leading dimensions, plus tiny, K = 0 and alpha = 0 ones."""
+    g = orc.rng(seed)
+    out = []
+    for i in range(n):
+        M = 32 * int(g.integers(1, 9))
+        hw = int(g.choice(sizes))
+        K = 9 * 16 * int(g.integers(1, 5))
+        alpha, beta = ((1.0, 0.0), (1.5, 0.5), (-1.0, 1.0))[i % 3]
+        out.append(Case("R", ("N", "T")[i % 2], tb, M, hw * hw, K, alpha, beta, seed=seed * 1000 + i, pad=(i % 3)))
+    out.append(Case("R", "N", tb, 1, 1, 1, 1.0, 0.0, seed=seed + 7))
+    out.append(Case("R", "T", tb, 33, 17, 0, 1.0, 0.5, seed=seed + 8))
+    out.append(Case("R", "N", tb, 40, 50, 60, 0.0, 0.25, seed=seed + 9))
+    out.append(Case("R", "T", tb, 129, 131, 77, 1.5, 0.5, seed=seed + 10, pad=1))
+    return out
+
+
+@pytest.mark.parametrize("mode", SPLIT_GROUPED)
+@pytest.mark.parametrize("variant", [7, 8])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_grouped_split_forms_split_b(mode, variant, tb):
+    """Grouped-only variants that split B in the core from its FP32 source
+    (no B planes): every B of the plan has one layout; the form's bound; a
+    captured plan replays bitwise identically."""
+    assert len(ug.grouped_variants(mode)) == 9 and ug.grouped_variants(mode)[variant].endswith("_split_b")
+    cases = [DevCase(c) for c in uniform_b_cases(12, 70 + variant + 3 * mode + (tb == "T"), tb)]
+    ug.force_variant(mode, variant)
+    try:
+        plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
+    finally:
+        ug.force_variant(mode, -1)
+    outs = []
+    for _ in range(2):
+        for d in cases:
+            d.reset()
+        plan.run(None)
+        hip.synchronize()
+        outs.append([d.dC.download() for d in cases])
+    for d, C in zip(cases, outs[0]):
+        ok, worst = d.case.judge(C, mode=mode)
+        assert ok, f"mode {mode} variant {variant} {d.case.ta}{tb} {d.case.M}x{d.case.N}x{d.case.K}: {worst:.3g}"
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    plan.destroy()
+
+
+def test_grouped_split_b_needs_one_b_layout():
+    """A split-B variant on a plan whose B layouts differ is refused
+    (UNSUPPORTED); the tuner skips such variants."""
+    cases = [DevCase(c) for c in uneven_cases(6, 88)]
+    ug.force_variant(ug.MODE_BF16X3, 7)
+    try:
+        with pytest.raises(ug.UgemmError) as e:
+            ug.GroupedPlan([d.problem() for d in cases], mode=ug.MODE_BF16X3, capture=False)
+        assert e.value.status == ug.ERR_UNSUPPORTED
+    finally:
+        ug.force_variant(ug.MODE_BF16X3, -1)
+    plan = ug.GroupedPlan([d.problem() for d in cases], mode=ug.MODE_BF16X3, capture=False)
+    assert plan.tune() < 7
+    plan.destroy()
+
+
 def test_grouped_split_form_graph_replays_and_tuning(tmp_path):
     """A captured BF16x3 plan replays bitwise identically; tuning picks one
     of the split-core variants and leaves results within the bound."""
@@ -197,7 +260,7 @@ def test_grouped_split_form_graph_replays_and_tuning(tmp_path):
     cases = [DevCase(c) for c in uneven_cases(20, 91, max_cin=128, sizes=(7, 14, 28, 56))]
     plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
     v = plan.tune()
-    assert 0 <= v < len(ug.variants(mode))
+    assert 0 <= v < len(ug.grouped_variants(mode))
     outs = []
     for _ in range(2):
         for d in cases:
@@ -276,7 +339,10 @@ def test_grouped_mode_variant_tables():
     L = ug.load()
     assert ug.grouped_variants(ug.MODE_FP32) == ug.grouped_variants()
     for mode in (ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3):
-        assert ug.grouped_variants(mode) == ug.variants(mode)
+        # the single-GEMM split-core table, then the grouped-only split-B variants
+        g, v = ug.grouped_variants(mode), ug.variants(mode)
+        assert g[:len(v)] == v and len(g) == len(v) + 2
+        assert g[len(v)] == v[1] + "_split_b"
     for mode in (ug.MODE_FP8_BLOCK, 42, -1):
         assert L.ugemm_hip_grouped_mode_variant_count(mode) == ug.ERR_ARG
     assert L.ugemm_hip_grouped_mode_variant_name(ug.MODE_FP32_MFMA, 5) is None

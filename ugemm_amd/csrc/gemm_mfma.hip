@@ -182,11 +182,10 @@ __global__ __launch_bounds__(256) void split_prep_kcont(const float *__restrict_
 // a row-major walk stored sixteen 64-byte pieces per instruction.  All four
 // chunks are loaded before any is split.
 template <int NP, class FMT>
-__global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__restrict__ X, long ld, int rows, int K,
-                                                              bool vec, typename FMT::E *__restrict__ out,
-                                                              long plane_stride, int rows_pad, int k_pad,
-                                                              const unsigned *__restrict__ amax) {
-    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 128, t = threadIdx.x;
+__device__ __forceinline__ void prep_kcont_tile(const float *__restrict__ X, long ld, int rows, int K, bool vec,
+                                                typename FMT::E *__restrict__ out, long plane_stride, int rows_pad,
+                                                int k_pad, const unsigned *__restrict__ amax, int r0, int k0) {
+    const int t = threadIdx.x;
     float x[4][8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -215,6 +214,15 @@ __global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__res
     }
 }
 
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__restrict__ X, long ld, int rows, int K,
+                                                              bool vec, typename FMT::E *__restrict__ out,
+                                                              long plane_stride, int rows_pad, int k_pad,
+                                                              const unsigned *__restrict__ amax) {
+    prep_kcont_tile<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, blockIdx.x * 64,
+                             blockIdx.y * 128);
+}
+
 // FP16X3, k-contiguous source with k_pad <= 8192: the row maximum and the
 // split in one pass.  Block r reads row r once into registers (at most 4
 // chunks of 8 per thread), reduces its largest magnitude (amax[r], as the
@@ -222,13 +230,13 @@ __global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__res
 // then scales by 2^-row_exp and writes the planes -- the operand is read
 // once instead of twice.
 template <int NP, class FMT>
-__global__ __launch_bounds__(256) void split_prep_kcont_rowscaled(const float *__restrict__ X, long ld, int rows,
-                                                                  int K, bool vec, typename FMT::E *__restrict__ out,
-                                                                  long plane_stride, int rows_pad, int k_pad,
-                                                                  unsigned *__restrict__ amax) {
+__device__ __forceinline__ void prep_kcont_rowscaled_row(const float *__restrict__ X, long ld, int rows, int K,
+                                                         bool vec, typename FMT::E *__restrict__ out,
+                                                         long plane_stride, int rows_pad, int k_pad,
+                                                         unsigned *__restrict__ amax, int r) {
     static_assert(FMT::SCALED, "row scales");
     constexpr int MAXC = 4;   // chunks per thread: k_pad <= 4 * 256 * 8
-    const int r = blockIdx.x, t = threadIdx.x;
+    const int t = threadIdx.x;
     const int nchunk = k_pad / 8;
     float x[MAXC][8];
     float m = 0.0f;
@@ -265,6 +273,14 @@ __global__ __launch_bounds__(256) void split_prep_kcont_rowscaled(const float *_
         for (int j = 0; j < 8; ++j) x[i][j] = ldexpf(x[i][j], e);
         write_planes<NP, FMT>(out + blocked_offset(r, c * 8, rows_pad), plane_stride, x[i]);
     }
+}
+
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_kcont_rowscaled(const float *__restrict__ X, long ld, int rows,
+                                                                  int K, bool vec, typename FMT::E *__restrict__ out,
+                                                                  long plane_stride, int rows_pad, int k_pad,
+                                                                  unsigned *__restrict__ amax) {
+    prep_kcont_rowscaled_row<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, blockIdx.x);
 }
 
 // Row-contiguous source (X[k*ld + r]): the 64 x 64 tile (rows r0.., k k0..)
@@ -323,24 +339,21 @@ __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict_
                              blockIdx.y * 64, tile);
 }
 
+// block_job[b]: the job of preparation block b (one load, where a search of
+// the job table took ~10 dependent loads per block).  A k-contiguous job
+// takes a 64-row x 128-k block per block (prep_kcont_tile: four 512-byte row
+// segments per load instruction, 256-byte runs per store), nbx blocks along
+// the rows; a row-contiguous job a 64 x 64 transposing tile.
 template <int NP, class FMT>
-__global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restrict__ jobs, int njobs) {
+__global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restrict__ jobs,
+                                                          const int *__restrict__ block_job) {
     __shared__ float tile[64][65];
-    // The job of this block: the last one whose first block is <= blockIdx.x.
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (jobs[mid].block0 <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const PrepJob j = jobs[lo];
+    const PrepJob j = jobs[block_job[blockIdx.x]];
     const int lb = (int)blockIdx.x - j.block0;
     typename FMT::E *out = static_cast<typename FMT::E *>(j.out);
     if (j.kcont) {
-        const long f = (long)lb * 256 + threadIdx.x;
-        if (f < (long)j.rows_pad * (j.k_pad / 8))
-            prep_kcont_chunk<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
-                                      j.amax, f);
+        prep_kcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
+                                 j.amax, (lb % j.nbx) * 64, (lb / j.nbx) * 128);
     } else {
         prep_rcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
                                  j.amax, (lb % j.nbx) * 64, (lb / j.nbx) * 64, tile);
@@ -449,14 +462,26 @@ __global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ 
 // Row maxima of the operands of a grouped FP16X3 plan: job j (PrepJob with
 // out = its maxima, nbx = K chunks per row (k-contiguous: 2048 wide) or per
 // 256-column block (row-contiguous: kAmaxRcontSpan deep)), blocks as in the kernels above.
-__global__ __launch_bounds__(256) void grouped_row_amax(const PrepJob *__restrict__ jobs, int njobs) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) / 2;
-        if (jobs[mid].block0 <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const PrepJob j = jobs[lo];
+// FP16X3 with k-contiguous op(A) (K <= 8192) and row-contiguous op(B): A's
+// fused row maximum + split (one block per padded row of A) and B's column
+// maxima (row_amax_rcont blocks, combined with atomicMax into zeroed maxima)
+// in one launch, so that the two passes over HBM overlap.  Blocks below
+// b_blocks scan B; the others split row blockIdx.x - b_blocks of A.
+template <int NP, class FMT>
+__global__ __launch_bounds__(256) void split_prep_rowscaled_a_amax_b(
+    const float *__restrict__ A, long lda, int rows_a, int K, bool vec_a, typename FMT::E *__restrict__ out_a,
+    long plane_a, int rows_pad_a, int k_pad, unsigned *__restrict__ amax_a, const float *__restrict__ B, long ldb,
+    int rows_b, bool vec_b, long kblocks_b, unsigned *__restrict__ amax_b, int b_blocks) {
+    if ((int)blockIdx.x < b_blocks)
+        row_amax_rcont_block(B, ldb, rows_b, K, vec_b, kblocks_b, amax_b, blockIdx.x);
+    else
+        prep_kcont_rowscaled_row<NP, FMT>(A, lda, rows_a, K, vec_a, out_a, plane_a, rows_pad_a, k_pad, amax_a,
+                                          (int)blockIdx.x - b_blocks);
+}
+
+__global__ __launch_bounds__(256) void grouped_row_amax(const PrepJob *__restrict__ jobs,
+                                                        const int *__restrict__ block_job) {
+    const PrepJob j = jobs[block_job[blockIdx.x]];
     const long lb = (long)blockIdx.x - j.block0;
     unsigned *amax = static_cast<unsigned *>(j.out);
     if (j.kcont) row_amax_kcont_block(j.X, j.ld, j.rows, j.K, j.vec != 0, j.nbx, amax, lb);
@@ -620,8 +645,9 @@ enum { FUSE_NONE = 0, FUSE_KCONT = 1, FUSE_RCONT = 2 };
 // Either way each split chunk lands where the plane copy would have put it
 // (zeros beyond M, N, K).
 // GR (grouped plans): block b computes tile gtiles[b] of problem
-// gprobs[gtiles[b].problem], whose planes, sizes, alpha, beta and C replace
-// the corresponding arguments (no split K, no epilogue, no fused operand).
+// gprobs[gtiles[b].problem], whose planes, sizes, alpha, beta and C (and,
+// with FB, B's FP32 source) replace the corresponding arguments (no split K,
+// no epilogue, A always from planes).
 template <class CF, int FA, int FB, bool GR = false>
 __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_, const typename CF::FMT::E *__restrict__ Ap_,
                                                        long a_plane_, int Mp_, const typename CF::FMT::E *__restrict__ Bp_,
@@ -630,14 +656,14 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                                                        const unsigned *__restrict__ amax_a_,
                                                        const unsigned *__restrict__ amax_b_, Epilogue epi,
                                                        const float *__restrict__ Af, long lda,
-                                                       const float *__restrict__ Bf, long ldb, int K, int vec_ab,
-                                                       int kt_per, float *__restrict__ ws,
+                                                       const float *__restrict__ Bf_, long ldb_, int K_,
+                                                       int vec_ab_, int kt_per, float *__restrict__ ws,
                                                        const GroupedSplitProblem *__restrict__ gprobs,
                                                        const GroupedSplitTile *__restrict__ gtiles) {
     using FMT = typename CF::FMT;
     using E = typename FMT::E;
     using V8 = typename FMT::V8;
-    static_assert(!GR || (FA == 0 && FB == 0), "grouped plans stream prepared planes");
+    static_assert(!GR || FA == 0, "grouped plans stream A's prepared planes");
     const GroupedSplitTile gt = GR ? gtiles[blockIdx.x] : GroupedSplitTile{0, 0, 0, 0};
     const GroupedSplitProblem *gp = GR ? gprobs + gt.problem : nullptr;
     const int M = GR ? gp->M : M_, N = GR ? gp->N : N_, Kp = GR ? gp->Kp : Kp_;
@@ -650,6 +676,11 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     const long ldc = GR ? gp->ldc : ldc_;
     const unsigned *__restrict__ amax_a = GR ? gp->amax_a : amax_a_;
     const unsigned *__restrict__ amax_b = GR ? gp->amax_b : amax_b_;
+    // B's FP32 source when the core splits it (grouped: per problem).
+    const float *__restrict__ Bf = GR ? gp->Bf : Bf_;
+    const long ldb = GR ? gp->ldb : ldb_;
+    const int K = GR ? gp->K : K_;
+    const int vec_ab = GR ? (gp->vec_b ? 2 : 0) : vec_ab_;
     constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, NP = CF::NP, FM = CF::FM, FN = CF::FN;
     constexpr int CPR = CF::CPR, STAGE = CF::STAGE, STAGES = CF::STAGES, PER_T = CF::PER_T;
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -1034,9 +1065,10 @@ static long round_up(long x, long a) { return (x + a - 1) / a * a; }
 // operand itself while staging it.
 template <int NP, class FMT>
 int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename FMT::E *out, long plane_stride,
-                 int rows_pad, int k_pad, unsigned *amax, hipStream_t s, bool split = true) {
+                 int rows_pad, int k_pad, unsigned *amax, hipStream_t s, bool split = true, bool amax_done = false) {
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
     if constexpr (FMT::SCALED) {
+        if (amax_done) goto planes;   // the row maxima were computed by an earlier launch
         if (split && kcont && rows > 0 && K > 0 && k_pad <= 8192) {
             // Row maxima and planes in one pass (one block per padded row).
             hipLaunchKernelGGL((split_prep_kcont_rowscaled<NP, FMT>), dim3((unsigned)rows_pad), dim3(256), 0, s, X,
@@ -1060,6 +1092,7 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
         UGEMM_HIP_TRY(hipGetLastError());
         return UGEMM_OK;
     }
+planes:
     if (kcont && rows_pad % 64 == 0 && (k_pad + 127) / 128 < 65536) {
         dim3 grid((unsigned)(rows_pad / 64), (unsigned)((k_pad + 127) / 128));
         hipLaunchKernelGGL((split_prep_kcont_tiles<NP, FMT>), grid, dim3(256), 0, s, X, ld, rows, K, vec, out,
@@ -1171,8 +1204,28 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
         hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((Mp + Np + 255) / 256, 1024)), dim3(256), 0,
                            s, amax_a, Mp + Np);
     }
-    rc = prep_operand<NP, FMT>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s, !fa);
-    if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s, !fb);
+    // FP16X3 NN (A k-contiguous with K <= 8192, B row-contiguous, both on
+    // planes): A's fused row maximum + split and B's column maxima in one
+    // launch, then B's split; otherwise one preparation per operand.
+    bool merged = false;
+    if constexpr (FMT::SCALED) {
+        if (!fa && !fb && !g.ta && !g.tb && g.M > 0 && g.N > 0 && g.K > 0 && Kp <= 8192) {
+            const long kblocks = (g.K + kAmaxRcontSpan - 1) / kAmaxRcontSpan, cblocks = (g.N + 255) / 256;
+            const bool vec_a = ugemm_aligned16(g.A) && g.lda % 4 == 0, vec_b = ugemm_aligned16(g.B) && g.ldb % 4 == 0;
+            hipLaunchKernelGGL((split_prep_rowscaled_a_amax_b<NP, FMT>), dim3((unsigned)(cblocks * kblocks + Mp)),
+                               dim3(256), 0, s, g.A, g.lda, g.M, g.K, vec_a, Ap, a_plane, (int)Mp, (int)Kp, amax_a,
+                               g.B, g.ldb, g.N, vec_b, kblocks, amax_b, (int)(cblocks * kblocks));
+            const hipError_t e = hipGetLastError();
+            rc = e == hipSuccess ? UGEMM_OK : hip_fail(e, "split preparation");
+            if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, false, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s,
+                                                true, true);
+            merged = true;
+        }
+    }
+    if (!merged) {
+        rc = prep_operand<NP, FMT>(g.A, g.lda, !g.ta, g.M, g.K, Ap, a_plane, (int)Mp, (int)Kp, amax_a, s, !fa);
+        if (!rc) rc = prep_operand<NP, FMT>(g.B, g.ldb, g.tb, g.N, g.K, Bp, b_plane, (int)Np, (int)Kp, amax_b, s, !fb);
+    }
     if (!rc) {
         const unsigned *ma = amax_a, *mb = amax_b;
 #define UGEMM_CORE(FA_, FB_) \
@@ -1329,8 +1382,8 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
 // Grouped split-form plans (grouped.hip): any variant (no operand is split
 // in the core here); the FP16 form adds the row-maximum pass.
 namespace {
-template <class CF>
-int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
+template <class CF, int FB = FUSE_NONE>
+int grouped_core(const PrepJob *jobs, const int *block_job, int prep_blocks, const GroupedSplitProblem *probs,
                  const GroupedSplitTile *tiles, int ntiles, hipStream_t s, const GroupedScales *sc) {
     using FMT = typename CF::FMT;
     if (FMT::SCALED) {
@@ -1338,12 +1391,15 @@ int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedS
         hipLaunchKernelGGL(fill_zero_u32, dim3((unsigned)std::min<long>((sc->count + 255) / 256, 4096)), dim3(256), 0,
                            s, sc->amax, sc->count);
         if (sc->blocks > 0)
-            hipLaunchKernelGGL(grouped_row_amax, dim3((unsigned)sc->blocks), dim3(256), 0, s, sc->jobs, sc->njobs);
+            hipLaunchKernelGGL(grouped_row_amax, dim3((unsigned)sc->blocks), dim3(256), 0, s, sc->jobs,
+                               sc->block_job);
     }
-    hipLaunchKernelGGL((grouped_split_prep<CF::NP, FMT>), dim3((unsigned)prep_blocks), dim3(256), 0, s, jobs, njobs);
+    if (prep_blocks > 0)
+        hipLaunchKernelGGL((grouped_split_prep<CF::NP, FMT>), dim3((unsigned)prep_blocks), dim3(256), 0, s, jobs,
+                           block_job);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "grouped split-form preparation");
-    auto kern = split_nt_core<CF, FUSE_NONE, FUSE_NONE, true>;
+    auto kern = split_nt_core<CF, FUSE_NONE, FB, true>;
     static const hipError_t attr = hipFuncSetAttribute((const void *)kern,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, CF::LDS_BYTES);
     if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute");
@@ -1357,18 +1413,37 @@ int grouped_core(const PrepJob *jobs, int njobs, int prep_blocks, const GroupedS
     return e == hipSuccess ? UGEMM_OK : hip_fail(e, "grouped split-form core");
 }
 
+// Grouped-only variants (kNumVariants + i): base variant split_b_base(np, i)
+// with B split in the core -- the 4-wave tiles with 128 B rows (each thread
+// stages two rows of a row-contiguous B): 128 x 128, and 256 x 128.
+constexpr int kNumSplitB = 2;
+constexpr int split_b_base(int np, int i) { return i == 0 ? 1 : (np == 3 ? 0 : 2); }
+
 template <int NP, class FMT = FmtBf16>
-int grouped_np(int variant, const PrepJob *jobs, int njobs, int prep_blocks, const GroupedSplitProblem *probs,
-               const GroupedSplitTile *tiles, int ntiles, hipStream_t s, const GroupedScales *sc) {
+int grouped_np(int variant, int b_kcont, const PrepJob *jobs, const int *block_job, int prep_blocks,
+               const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s,
+               const GroupedScales *sc) {
     using T = Table<NP, FMT>;
+#define UGEMM_SPLIT_B(CFG)                                                                                    \
+    (b_kcont ? grouped_core<CFG, FUSE_KCONT>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc)           \
+             : grouped_core<CFG, FUSE_RCONT>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc))
+    static_assert(split_b_base(NP, 0) == 1 && split_b_base(NP, 1) == (NP == 3 ? 0 : 2), "split-B table");
     switch (variant) {
-        case 0: return grouped_core<typename T::V0>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 1: return grouped_core<typename T::V1>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 2: return grouped_core<typename T::V2>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 3: return grouped_core<typename T::V3>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 4: return grouped_core<typename T::V4>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 5: return grouped_core<typename T::V5>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case 6: return grouped_core<typename T::V6>(jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case kNumVariants + 0: return UGEMM_SPLIT_B(typename T::V1);
+        case kNumVariants + 1:
+            if constexpr (NP == 3) return UGEMM_SPLIT_B(typename T::V0);
+            else return UGEMM_SPLIT_B(typename T::V2);
+        default: break;
+    }
+#undef UGEMM_SPLIT_B
+    switch (variant) {
+        case 0: return grouped_core<typename T::V0>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 1: return grouped_core<typename T::V1>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 2: return grouped_core<typename T::V2>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 3: return grouped_core<typename T::V3>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 4: return grouped_core<typename T::V4>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 5: return grouped_core<typename T::V5>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 6: return grouped_core<typename T::V6>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         default: return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     }
 }
@@ -1388,7 +1463,30 @@ void tile_of(int variant, int *bm, int *bn) {
 }
 }  // namespace
 
+int mfma_grouped_variant_count(int mode) { return planes_of(mode) ? kNumVariants + kNumSplitB : 0; }
+
+int mfma_grouped_splits_b(int mode, int variant) {
+    return planes_of(mode) && variant >= kNumVariants && variant < kNumVariants + kNumSplitB ? 1 : 0;
+}
+
+const char *mfma_grouped_variant_name(int mode, int variant) {
+    if (!planes_of(mode) || variant < 0 || variant >= kNumVariants + kNumSplitB) return nullptr;
+    if (variant < kNumVariants) return mfma_variant_name(mode, variant);
+    // Formatted once per (mode, variant); function-local static initialisation is thread-safe.
+    static const std::string names[UGEMM_MODE_COUNT][kNumSplitB] = {
+#define UGEMM_N(m, i) (std::string(mfma_variant_name(m, split_b_base(planes_of(m), i)) \
+                                    ? mfma_variant_name(m, split_b_base(planes_of(m), i)) : "") + "_split_b")
+        {UGEMM_N(0, 0), UGEMM_N(0, 1)}, {UGEMM_N(1, 0), UGEMM_N(1, 1)}, {UGEMM_N(2, 0), UGEMM_N(2, 1)},
+        {UGEMM_N(3, 0), UGEMM_N(3, 1)}, {UGEMM_N(4, 0), UGEMM_N(4, 1)}, {UGEMM_N(5, 0), UGEMM_N(5, 1)},
+        {UGEMM_N(6, 0), UGEMM_N(6, 1)}
+#undef UGEMM_N
+    };
+    return names[mode][variant - kNumVariants].c_str();
+}
+
 int mfma_grouped_tile(int mode, int variant, int *bm, int *bn) {
+    if (variant >= kNumVariants && variant < kNumVariants + kNumSplitB)
+        variant = split_b_base(planes_of(mode), variant - kNumVariants);
     if (variant < 0 || variant >= kNumVariants) return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     switch (mode) {
         case UGEMM_MODE_BF16X1: tile_of<1>(variant, bm, bn); return UGEMM_OK;
@@ -1399,17 +1497,21 @@ int mfma_grouped_tile(int mode, int variant, int *bm, int *bn) {
     }
 }
 
-int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
+int launch_mfma_grouped(int mode, int variant, int b_kcont, const PrepJob *jobs, const int *block_job,
+                        int prep_blocks,
                         const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s,
                         const GroupedScales *sc) {
     if (ntiles == 0) return UGEMM_OK;
     switch (mode) {
-        case UGEMM_MODE_BF16X1: return grouped_np<1>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case UGEMM_MODE_BF16X3: return grouped_np<2>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
-        case UGEMM_MODE_BF16X6: return grouped_np<3>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_BF16X1:
+            return grouped_np<1>(variant, b_kcont, jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_BF16X3:
+            return grouped_np<2>(variant, b_kcont, jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case UGEMM_MODE_BF16X6:
+            return grouped_np<3>(variant, b_kcont, jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         case UGEMM_MODE_FP16X3:
             if (!sc) return fail(UGEMM_ERR_ARG, "grouped FP16X3: no row-scale jobs");
-            return grouped_np<2, FmtF16>(variant, jobs, njobs, prep_blocks, probs, tiles, ntiles, s, sc);
+            return grouped_np<2, FmtF16>(variant, b_kcont, jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         default: return fail(UGEMM_ERR_UNSUPPORTED, "grouped: mode %d not provided", mode);
     }
 }

@@ -226,7 +226,7 @@ int grouped_variant_count(int mode) {
     if (mode == UGEMM_MODE_FP32 || mode == UGEMM_MODE_FP32_MFMA) return kNumVariants;
     if (mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6 ||
         mode == UGEMM_MODE_FP16X3)
-        return mfma_variant_count(mode);
+        return mfma_grouped_variant_count(mode);
     return 0;
 }
 
@@ -257,6 +257,9 @@ struct ugemm_grouped_plan {
     GroupedSplitProblem *d_sprobs = nullptr;
     GroupedSplitTile *d_stiles = nullptr;
     int njobs = 0, prep_blocks = 0;
+    int *d_block_job = nullptr;    // preparation block -> job
+    int *d_ablock_job = nullptr;   // row-maximum block -> job (FP16X3)
+    int b_kcont = 0;            // variants that split B in the core: every B is k-contiguous ('T')
     // FP16X3: every operand row's maximum (the row scales), and its jobs.
     unsigned *d_amax = nullptr;
     PrepJob *d_ajobs = nullptr;
@@ -283,8 +286,12 @@ void release_schedule(ugemm_grouped_plan *p) {
     hipFree(p->d_stiles);
     hipFree(p->d_amax);
     hipFree(p->d_ajobs);
+    hipFree(p->d_block_job);
+    hipFree(p->d_ablock_job);
     p->d_amax = nullptr;
     p->d_ajobs = nullptr;
+    p->d_block_job = nullptr;
+    p->d_ablock_job = nullptr;
     p->d_tiles = nullptr;
     p->d_queues = nullptr;
     p->d_heads = nullptr;
@@ -297,9 +304,9 @@ void release_schedule(ugemm_grouped_plan *p) {
 int launch_plan_kernel(ugemm_grouped_plan *p, hipStream_t s) {
     if (p->grid == 0) return UGEMM_OK;
     if (p->split) {
-        const GroupedScales sc{p->d_amax, p->amax_count, p->d_ajobs, p->najobs, p->amax_blocks};
-        return launch_mfma_grouped(p->mode, p->variant, p->d_jobs, p->njobs, p->prep_blocks, p->d_sprobs,
-                                   p->d_stiles, p->ntiles, s, p->d_amax ? &sc : nullptr);
+        const GroupedScales sc{p->d_amax, p->amax_count, p->d_ajobs, p->d_ablock_job, p->amax_blocks};
+        return launch_mfma_grouped(p->mode, p->variant, p->b_kcont, p->d_jobs, p->d_block_job, p->prep_blocks,
+                                   p->d_sprobs, p->d_stiles, p->ntiles, s, p->d_amax ? &sc : nullptr);
     }
     const VariantOps ops = variant_ops(p->mode, p->variant);
     hipLaunchKernelGGL(grouped_reset_heads, dim3(1), dim3(64), 0, s, p->d_heads);
@@ -396,6 +403,20 @@ int capture_plan(ugemm_grouped_plan *p) {
 // zero planes, so the core computes beta * C), the preparation jobs, and the
 // tiles in grouped order sorted by K, longest first (the hardware dispatches
 // blocks in order, which balances them as they finish).
+// The job of every block of a job grid (jobs in order of their first block).
+int upload_block_jobs(const std::vector<PrepJob> &jobs, long nblocks, int **d_map) {
+    *d_map = nullptr;
+    if (nblocks <= 0) return UGEMM_OK;
+    std::vector<int> map((size_t)nblocks);
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const long b1 = i + 1 < jobs.size() ? jobs[i + 1].block0 : nblocks;
+        for (long b = jobs[i].block0; b < b1; ++b) map[(size_t)b] = (int)i;
+    }
+    UGEMM_HIP_TRY(hipMalloc(d_map, map.size() * sizeof(int)));
+    UGEMM_HIP_TRY(hipMemcpy(*d_map, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+    return UGEMM_OK;
+}
+
 int build_split_schedule(ugemm_grouped_plan *p, int v) {
     release_schedule(p);
     p->variant = v;
@@ -404,6 +425,19 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     if (rc) return rc;
     const int np = p->mode == UGEMM_MODE_BF16X1 ? 1 : (p->mode == UGEMM_MODE_BF16X6 ? 3 : 2);
     const bool scaled = p->mode == UGEMM_MODE_FP16X3;
+    // Variants that split B in the core read every B in one layout.
+    const bool split_b = mfma_grouped_splits_b(p->mode, v) != 0;
+    p->b_kcont = 0;
+    if (split_b) {
+        int n_t = 0, n_n = 0;
+        for (const DevProblem &q : p->probs) {
+            if (q.M == 0 || q.N == 0) continue;
+            ((q.layout & 2) ? n_t : n_n)++;
+        }
+        if (n_t && n_n)
+            return fail(UGEMM_ERR_UNSUPPORTED, "grouped variant %d splits B in the core: B layouts differ", v);
+        p->b_kcont = n_t ? 1 : 0;
+    }
     std::vector<GroupedSplitProblem> sp;
     std::vector<PrepJob> jobs, ajobs;
     long acount = 0, ablocks = 0;   // row maxima, row-maximum blocks (FP16X3)
@@ -447,7 +481,7 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
         j.block0 = (int)blocks;
         j.nbx = rows_pad / 64;
-        blocks += kcont ? ((long)rows_pad * (k_pad / 8) + 255) / 256 : (long)(rows_pad / 64) * (k_pad / 64);
+        blocks += (long)(rows_pad / 64) * (kcont ? (k_pad + 127) / 128 : k_pad / 64);
         jobs.push_back(j);
     };
     for (int i = 0; i < (int)p->probs.size(); ++i) {
@@ -458,7 +492,7 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         const long Kp = std::max<long>(64, (q.K + 63) / 64 * 64);
         const long a_plane = Mp * Kp, b_plane = Np * Kp;
         const long ao = total, bo = total + np * a_plane;
-        total = bo + np * b_plane;
+        total = bo + (split_b ? 0 : np * b_plane);
         const bool ta = (q.layout & 1) != 0, tb = (q.layout & 2) != 0;
         const long am_a = acount, am_b = acount + Mp;
         if (scaled) {
@@ -467,8 +501,16 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
             add_amax_job(q.B, q.ldb, tb, q.N, q.K, am_b);
         }
         add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane, am_a);
-        add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane, am_b);
+        if (!split_b) add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane, am_b);
         GroupedSplitProblem g{};
+        if (split_b) {
+            // FP32 B read by the core; K == 0 (alpha == 0) reads nothing.
+            g.Bf = q.B;
+            g.ldb = q.ldb;
+            g.K = q.K;
+            g.vec_b = tb ? (q.K > 0 && ugemm_aligned16(q.B) && q.ldb % 4 == 0)
+                         : ((reinterpret_cast<uintptr_t>(q.B) & 7) == 0 && q.ldb % 2 == 0);
+        }
         g.amax_a = scaled ? (const unsigned *)(intptr_t)am_a : nullptr;
         g.amax_b = scaled ? (const unsigned *)(intptr_t)am_b : nullptr;
         g.Ap = (const void *)(intptr_t)ao;
@@ -508,7 +550,7 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     for (PrepJob &j : jobs) j.out = base + (intptr_t)j.out * 2;
     for (GroupedSplitProblem &g : sp) {
         g.Ap = base + (intptr_t)g.Ap * 2;
-        g.Bp = base + (intptr_t)g.Bp * 2;
+        g.Bp = split_b ? nullptr : base + (intptr_t)g.Bp * 2;
     }
     if (scaled) {
         // Row maxima: rebase the offsets on the buffer's address.
@@ -526,8 +568,12 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         if (!ajobs.empty()) {
             UGEMM_HIP_TRY(hipMalloc(&p->d_ajobs, ajobs.size() * sizeof(PrepJob)));
             UGEMM_HIP_TRY(hipMemcpy(p->d_ajobs, ajobs.data(), ajobs.size() * sizeof(PrepJob), hipMemcpyHostToDevice));
+            rc = upload_block_jobs(ajobs, ablocks, &p->d_ablock_job);
+            if (rc) return rc;
         }
     }
+    rc = upload_block_jobs(jobs, blocks, &p->d_block_job);
+    if (rc) return rc;
     std::vector<GroupedSplitTile> flat(tiles.size());
     for (size_t i = 0; i < tiles.size(); ++i) flat[i] = tiles[i].t;
     UGEMM_HIP_TRY(hipMalloc(&p->d_jobs, jobs.size() * sizeof(PrepJob)));
@@ -772,7 +818,7 @@ const char *ugemm_hip_grouped_mode_variant_name(int mode, int v) {
     if (v < 0 || v >= grouped_variant_count(mode)) return nullptr;
     if (mode == UGEMM_MODE_FP32) return kGroupedNames[v];
     if (mode == UGEMM_MODE_FP32_MFMA) return kGroupedM32Names[v];
-    return mfma_variant_name(mode, v);
+    return mfma_grouped_variant_name(mode, v);
 }
 
 // Times each tile variant on the plan's own problems (C is overwritten, as by
@@ -823,6 +869,10 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
     const int nv = grouped_variant_count(p->mode);
     for (int v = 0; v < nv && !rc; ++v) {
         rc = p->split ? build_split_schedule(p, v) : build_schedule(p, v);
+        if (rc == UGEMM_ERR_UNSUPPORTED) {   // a variant this plan's layouts do not allow
+            rc = UGEMM_OK;
+            continue;
+        }
         if (rc) break;
         launch_plan_kernel(p, s);   // warm-up
         hipEventRecord(e0, s);

@@ -49,14 +49,19 @@ struct GroupedSplitProblem {
     float *C;
     long ldc;
     const unsigned *amax_a, *amax_b;   // FP16X3: per-row maxima (Mp, Np entries)
+    // Variants that split B in the core (no B planes): B's FP32 source, its
+    // leading dimension, the true K, and whether vector loads are allowed.
+    const float *Bf;
+    long ldb;
+    int K, vec_b;
 };
 struct GroupedSplitTile {
     int problem, m0, n0, pad;
 };
 // One operand of a grouped split-form plan to prepare: its FP32 source and
 // its planes, and the first block of the job in the preparation grid.
-// k-contiguous jobs take one 8-k chunk per thread (256 per block);
-// row-contiguous jobs a 64 x 64 tile per block (nbx tiles along the rows).
+// k-contiguous jobs take a 64-row x 128-k block per block, row-contiguous
+// jobs a 64 x 64 tile per block (nbx blocks along the rows either way).
 struct PrepJob {
     const float *X;
     long ld;
@@ -67,13 +72,14 @@ struct PrepJob {
     int block0, nbx;
     const unsigned *amax;   // FP16X3: the operand's row maxima (row scales), else null
 };
-// FP16X3 grouped plans: the row-maximum jobs (out = maxima) and the maxima
-// buffer of all problems, zeroed before the pass.
+// FP16X3 grouped plans: the row-maximum jobs (out = maxima), the job of
+// each of their blocks, and the maxima buffer of all problems, zeroed before
+// the pass.
 struct GroupedScales {
     unsigned *amax;
     long count;
     const PrepJob *jobs;
-    int njobs;
+    const int *block_job;
     long blocks;
 };
 
@@ -124,8 +130,18 @@ int mfma_pick_variant(const Gemm<float> &g, int mode);
 int launch_mfma(const Gemm<float> &g, int mode, int variant, hipStream_t s);
 // Grouped split-form plans: tile shape of a variant, and the launch of
 // the preparation (jobs) then the core (one block per tile).
+// Grouped split-form variants: the single-GEMM table (planes of A and B
+// prepared) plus variants that split B in the core from its FP32 source.
+int mfma_grouped_variant_count(int mode);
+const char *mfma_grouped_variant_name(int mode, int variant);
+// 1 when the grouped variant splits B in the core (then every problem's B
+// must have one layout: row-contiguous, 'N', or k-contiguous, 'T').
+int mfma_grouped_splits_b(int mode, int variant);
 int mfma_grouped_tile(int mode, int variant, int *bm, int *bn);
-int launch_mfma_grouped(int mode, int variant, const PrepJob *jobs, int njobs, int prep_blocks,
+// b_kcont (variants that split B in the core): B is k-contiguous ('T').
+// block_job[b]: the preparation job of block b.
+int launch_mfma_grouped(int mode, int variant, int b_kcont, const PrepJob *jobs, const int *block_job,
+                        int prep_blocks,
                         const GroupedSplitProblem *probs, const GroupedSplitTile *tiles, int ntiles, hipStream_t s,
                         const GroupedScales *scales = nullptr);
 // Convolution in a split form: g.A = weights (g.M x g.K, k-contiguous), g.B

@@ -13,9 +13,9 @@
 //
 // Planes use the k-tile blocked layout of the split-BF16 path with 128-byte
 // rows (one K tile = one scale block): byte (r, k) of an operand with rows_pad
-// rows is at ((k / 128) * rows_pad + r) * 128 + ((k % 128) / 16 ^ (r & 7)) * 16
-// + k % 16.  The XOR makes the fragment reads (16 rows, one 16-byte chunk
-// each) conflict-free and lets the core copy a stage verbatim.
+// rows is at ((k / 128) * rows_pad + r) * 128 + ((k % 128) / 16 ^ fp8_swz(r)) * 16
+// + k % 16.  The XOR makes the fragment reads conflict-free (see fp8_swz)
+// and lets the core copy a stage verbatim.
 // Fragment layout of the 16x16x128 fp8 MFMA: lane l holds row l%16 and k
 // 32*(l/16) .. +31 of A, the same k of column l%16 of B; the per-lane scale
 // operand is the E8M0 exponent of the block those elements belong to (one
@@ -32,8 +32,17 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int SB = 128;   // scale block edge = K tile
 
+// Chunk swizzle of row r: chunk c of a 128-byte plane row is stored at
+// c ^ fp8_swz(r).  A fragment read serves 16 lanes at a time as four rows x
+// four lane quarters (lanes l..l+3, l+16.., l+32.., l+48..) over 64 banks:
+// every quarter reads chunk 2q (+1), so the two even (and the two odd) rows
+// of each group of four must XOR with values of opposite parity to land in
+// distinct 16-byte bank groups.  r & 7 alone gave rows 4a and 4a + 2 the same
+// parity (measured: SQ_LDS_BANK_CONFLICT about 3 cycles per LDS instruction).
+__device__ __forceinline__ int fp8_swz(int r) { return (r & 7) ^ ((r >> 1) & 1); }
+
 __device__ __forceinline__ long fp8_blocked_offset(int r, int k, int rows_pad) {
-    return ((long)(k >> 7) * rows_pad + r) * 128 + ((((k >> 4) & 7) ^ (r & 7)) << 4);
+    return ((long)(k >> 7) * rows_pad + r) * 128 + ((((k >> 4) & 7) ^ fp8_swz(r)) << 4);
 }
 
 // Power-of-two exponent for a block maximum (see the header comment).
@@ -52,18 +61,34 @@ __device__ __forceinline__ int pack4_e4m3(float a, float b, float c, float d) {
     return w;
 }
 
-// One workgroup per 128 x 128 block.  Source element (r, k): KCONT ? X[r*ld+k]
-// : X[k*ld+r]; zeros outside rows x K.  Writes the quantised block and its
-// exponent exps[(r / 128) * kblocks + k / 128].
+// One operand to quantise: source element (r, k) is kcont ? X[r*ld+k] :
+// X[k*ld+r]; zeros outside rows x K.  rblocks = rows_pad / 128 blocks of
+// rows; exps[(r / 128) * kblocks + k / 128] receives each block's exponent.
+struct Fp8Operand {
+    const float *X;
+    long ld;
+    int rows, K, vec, kcont;
+    unsigned char *out;
+    int rows_pad, rblocks;
+    int *exps;
+};
+
+// LDS of one 128 x 128 block: k-contiguous sources quantise into byte rows
+// [r][k] (pitch 144 bytes); row-contiguous ones into words [k][r/4] of four
+// r-consecutive bytes (pitch 34 words: the 32 lanes of a word store and the
+// two 16-line-apart rows a wave reads back land in different banks), and
+// every output chunk is assembled by 4 x 4 byte transposes of 16 words.
+constexpr int kF8Pitch = 128 + 16, kF8WPitch = 34;
+constexpr int kF8Lds = 128 * (kF8Pitch > 4 * kF8WPitch ? kF8Pitch : 4 * kF8WPitch);
+
 template <bool KCONT>
-__global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ X, long ld, int rows, int K, bool vec,
-                                                      unsigned char *__restrict__ out, int rows_pad,
-                                                      int *__restrict__ exps, int kblocks) {
-    __shared__ float wmax[4];
-    __shared__ __attribute__((aligned(16))) unsigned char tile[128][128 + 16];
-    const int rb = blockIdx.x, kb = blockIdx.y;
+__device__ __forceinline__ void fp8_quantise_block(const Fp8Operand &op, int rb, int kb, int kblocks,
+                                                   unsigned char *smem, float *wmax) {
     const int r0 = rb * SB, k0 = kb * SB;
     const int t = threadIdx.x;
+    const float *__restrict__ X = op.X;
+    const long ld = op.ld;
+    const int rows = op.rows, K = op.K;
     // The block's source is 128 lines (rows for KCONT, k for the other
     // layout) of 128 contiguous floats.  Load q of a thread takes line
     // q*8 + t/32, floats (t%32)*4 .. +3: each load instruction of a wave
@@ -77,7 +102,7 @@ __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ 
         const int r = KCONT ? r0 + line : r0 + cl, k = KCONT ? k0 + cl : k0 + line;
         const float *p = KCONT ? X + (long)r * ld + k : X + (long)k * ld + r;
         const bool full = KCONT ? (r < rows && k + 4 <= K) : (k < K && r + 4 <= rows);
-        if (full && vec) {
+        if (full && op.vec) {
             const float4 x = *reinterpret_cast<const float4 *>(p);
             v[q * 4 + 0] = x.x; v[q * 4 + 1] = x.y; v[q * 4 + 2] = x.z; v[q * 4 + 3] = x.w;
         } else {
@@ -102,27 +127,56 @@ __global__ __launch_bounds__(256) void fp8_block_prep(const float *__restrict__ 
     const float bmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
     const int e = block_exponent(bmax);
     const float inv = ldexpf(1.0f, -e);
-    if (t == 0) exps[(long)rb * kblocks + kb] = e;
-    // Quantise into the LDS tile [r][k] (bytes), then write rows out.
+    if (t == 0) op.exps[(long)rb * kblocks + kb] = e;
+    // Quantise into LDS; each packed word holds 4 consecutive elements of
+    // its line (k for KCONT, r otherwise).
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int w = pack4_e4m3(v[q * 4] * inv, v[q * 4 + 1] * inv, v[q * 4 + 2] * inv, v[q * 4 + 3] * inv);
         const int line = q * 8 + sl;
-        if (KCONT) {
-            *reinterpret_cast<int *>(&tile[line][cl]) = w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) tile[cl + j][line] = (unsigned char)((unsigned)w >> (8 * j));
-        }
+        if (KCONT) *reinterpret_cast<int *>(smem + line * kF8Pitch + cl) = w;
+        else reinterpret_cast<int *>(smem)[line * kF8WPitch + (t & 31)] = w;
     }
     __syncthreads();
-    // 128 rows x 8 chunks of 16 bytes = 1024 chunks, 4 per thread.
+    if (KCONT) {
+        // 128 rows x 8 chunks of 16 bytes = 1024 chunks, 4 per thread.
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int f = t + i * 256, r = f >> 3, c = f & 7;
-        const uint4 x = *reinterpret_cast<const uint4 *>(&tile[r][c * 16]);
-        *reinterpret_cast<uint4 *>(out + fp8_blocked_offset(r0 + r, k0 + c * 16, rows_pad)) = x;
+        for (int i = 0; i < 4; ++i) {
+            const int f = t + i * 256, r = f >> 3, c = f & 7;
+            const uint4 x = *reinterpret_cast<const uint4 *>(smem + r * kF8Pitch + c * 16);
+            *reinterpret_cast<uint4 *>(op.out + fp8_blocked_offset(r0 + r, k0 + c * 16, op.rows_pad)) = x;
+        }
+    } else {
+        // Thread t: rows 4*(t%32) .. +3, k chunk t/32 -- 16 words of 4 rows,
+        // transposed 4 x 4 bytes at a time into the 4 rows' 16-byte chunks.
+        const int rq = t & 31, kc = t >> 5;
+        const int *wt = reinterpret_cast<const int *>(smem);
+        unsigned a[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] = (unsigned)wt[(kc * 16 + i) * kF8WPitch + rq];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned o[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                o[g] = ((a[4 * g] >> (8 * j)) & 0xffu) | (((a[4 * g + 1] >> (8 * j)) & 0xffu) << 8) |
+                       (((a[4 * g + 2] >> (8 * j)) & 0xffu) << 16) | (((a[4 * g + 3] >> (8 * j)) & 0xffu) << 24);
+            *reinterpret_cast<uint4 *>(op.out + fp8_blocked_offset(r0 + 4 * rq + j, k0 + kc * 16, op.rows_pad)) =
+                make_uint4(o[0], o[1], o[2], o[3]);
+        }
     }
+}
+
+// Both operands in one launch: blocks (x, y) with x < a.rblocks quantise
+// block (x, y) of A, the others block (x - a.rblocks, y) of B.
+__global__ __launch_bounds__(256) void fp8_prep_operands(Fp8Operand a, Fp8Operand b, int kblocks) {
+    __shared__ float wmax[4];
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kF8Lds];
+    const bool is_b = (int)blockIdx.x >= a.rblocks;
+    const Fp8Operand &op = is_b ? b : a;
+    const int rb = is_b ? (int)blockIdx.x - a.rblocks : (int)blockIdx.x;
+    if (op.kcont) fp8_quantise_block<true>(op, rb, blockIdx.y, kblocks, smem, wmax);
+    else fp8_quantise_block<false>(op, rb, blockIdx.y, kblocks, smem, wmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,8 +252,8 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
     const int a_blk = (m0 + wm * CF::WTM) / SB, b_blk = (n0 + wn * CF::WTN) / SB;
     // Fragment of plane row `row` (16 rows x 128 k, 32 bytes per lane).
     auto frag = [&](const unsigned char *base, int row) {
-        const u32x4 lo = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq) ^ (row & 7)) << 4));
-        const u32x4 hi = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq + 1) ^ (row & 7)) << 4));
+        const u32x4 lo = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq) ^ fp8_swz(row)) << 4));
+        const u32x4 hi = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq + 1) ^ fp8_swz(row)) << 4));
         return int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
     };
     for (int kt = 0; kt < kblocks; ++kt) {
@@ -293,18 +347,10 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
 
 long round_up(long x, long a) { return (x + a - 1) / a * a; }
 
-int prep(const float *X, long ld, bool kcont, int rows, int K, unsigned char *out, int rows_pad, int k_pad, int *exps,
-         hipStream_t s) {
-    const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
-    dim3 grid((unsigned)(rows_pad / SB), (unsigned)(k_pad / SB));
-    if (kcont)
-        hipLaunchKernelGGL(fp8_block_prep<true>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, rows_pad, exps,
-                           k_pad / SB);
-    else
-        hipLaunchKernelGGL(fp8_block_prep<false>, grid, dim3(256), 0, s, X, ld, rows, K, vec, out, rows_pad, exps,
-                           k_pad / SB);
-    UGEMM_HIP_TRY(hipGetLastError());
-    return UGEMM_OK;
+Fp8Operand operand(const float *X, long ld, bool kcont, int rows, int K, unsigned char *out, int rows_pad,
+                   int *exps) {
+    return Fp8Operand{X, ld, rows, K, (ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0, kcont ? 1 : 0, out, rows_pad,
+                      rows_pad / SB, exps};
 }
 
 template <class CF>
@@ -313,6 +359,7 @@ int run(const Gemm<float> &g, hipStream_t s) {
     const int tiles_m = (int)(Mp / CF::BM), tiles_n = (int)(Np / CF::BN);
     if ((long)tiles_m * tiles_n > 0x7fffffffL) return fail(UGEMM_ERR_ARG, "too many tiles");
     const long kb = Kp / SB;
+    if (kb > 65535) return fail(UGEMM_ERR_UNSUPPORTED, "fp8: K = %d exceeds %d", g.K, 65535 * SB);
     const size_t a_bytes = (size_t)Mp * Kp, b_bytes = (size_t)Np * Kp;
     const size_t e_bytes = (size_t)((Mp + Np) / SB) * kb * sizeof(int);
     void *ws = nullptr;
@@ -320,8 +367,14 @@ int run(const Gemm<float> &g, hipStream_t s) {
     if (rc) return rc;
     unsigned char *Aq = (unsigned char *)ws, *Bq = Aq + a_bytes;
     int *ea = (int *)(Bq + b_bytes), *eb = ea + (Mp / SB) * kb;
-    rc = prep(g.A, g.lda, !g.ta, g.M, g.K, Aq, (int)Mp, (int)Kp, ea, s);
-    if (!rc) rc = prep(g.B, g.ldb, g.tb, g.N, g.K, Bq, (int)Np, (int)Kp, eb, s);
+    {
+        const Fp8Operand oa = operand(g.A, g.lda, !g.ta, g.M, g.K, Aq, (int)Mp, ea);
+        const Fp8Operand ob = operand(g.B, g.ldb, g.tb, g.N, g.K, Bq, (int)Np, eb);
+        hipLaunchKernelGGL(fp8_prep_operands, dim3((unsigned)(oa.rblocks + ob.rblocks), (unsigned)kb), dim3(256), 0, s,
+                           oa, ob, (int)kb);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = hip_fail(e, "fp8 quantisation");
+    }
     if (!rc) {
         auto kern = fp8_scaled_core<CF>;
         // Once per kernel instantiation (thread-safe static initialisation).

@@ -15,6 +15,7 @@
 
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -94,7 +95,9 @@ extern "C" {
 
 int ugemm_hip_force_variant(int mode, int variant) {
     if (mode < -1 || mode >= UGEMM_MODE_COUNT) return fail(UGEMM_ERR_ARG, "unknown mode %d", mode);
-    if (variant >= ugemm_hip_variant_count(mode)) return fail(UGEMM_ERR_ARG, "mode %d has no variant %d", mode, variant);
+    // Single-GEMM variants, or the (possibly longer) table of the mode's grouped plans.
+    const int nv = std::max(ugemm_hip_variant_count(mode), mode >= 0 ? grouped_variant_count(mode) : 0);
+    if (variant >= nv) return fail(UGEMM_ERR_ARG, "mode %d has no variant %d", mode, variant);
     t_forced_p1[slot(mode)] = variant < 0 ? 0 : variant + 1;
     return UGEMM_OK;
 }
