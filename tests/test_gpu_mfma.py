@@ -59,6 +59,30 @@ def test_split_every_variant(mode, variant, ta, tb):
         ug.force_variant(mode, -1)
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("shape,splits", [((1000, 1100, 1200, "N", "N"), 1), ((777, 513, 333, "T", "T"), 1),
+                                          ((600, 530, 4096, "T", "N"), 3), ((530, 600, 40, "N", "T"), 1)])
+def test_split_variants_bitwise_identical(mode, shape, splits):
+    """The variants differ in tiles, waves and the order of their LDS reads,
+    never in the MFMAs an accumulator receives or their order: at one split
+    count, every variant gives the same bits."""
+    M, N, K, ta, tb = shape
+    case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=zlib.crc32(f"bitwise{mode}{shape}".encode()))
+    outs = []
+    for v in range(len(ug.variants(mode))):
+        ug.force_variant(mode, v)
+        ug.force_splits(mode, splits)   # the same split count for every variant
+        try:
+            outs.append(run_host(case, mode))
+        finally:
+            ug.force_variant(mode, -1)
+            ug.force_splits(mode, 0)
+    ok, worst = case.judge(outs[0], mode=mode)
+    assert ok, f"{worst:.3g}"
+    for v in range(1, len(outs)):
+        assert np.array_equal(outs[0].view(np.uint32), outs[v].view(np.uint32)), f"variant {v} differs from 0"
+
+
 @pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
 @pytest.mark.parametrize("variant", [4, 6])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])

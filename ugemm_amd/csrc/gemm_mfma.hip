@@ -604,12 +604,24 @@ __device__ __forceinline__ void static_for(F &&f) {
 // product instead of running between the MFMAs and the barrier: the loads
 // get a full step of latency and no wave stalls on a burst of stores in
 // front of the barrier.  It needs the registers of 2 waves per SIMD.
+// STREAM (required by ILV): each A fragment is read two MFMA groups (of FN
+// MFMAs) ahead of its use, and the next product's B fragments during the
+// current product's last groups, in an order that scheduling barriers fix.
+// (Left to itself the compiler issued a product's reads in one burst and
+// then waited on them.)  Every variant issues the same MFMAs per accumulator
+// in the same order: results are bitwise identical across them.
+// EARLY (with STREAM, for NP >= 2, whose last product reuses planes held in
+// registers): the block barrier moves from the end of a K tile to the start
+// of its last product.  The stage copy is dealt over the products before the
+// barrier, so after it the last product's MFMAs cover the reads of the next
+// K tile's first fragments (from the other buffer) instead of the tile
+// starting on a burst of reads.
 template <int BM_, int BN_, int BK_, int NP_, class FMT_ = FmtBf16, int WGM_ = 2, int WGN_ = 2, bool PHASED_ = false,
-          bool ILV_ = false>
+          bool ILV_ = false, bool STREAM_ = false, bool EARLY_ = false>
 struct CoreCfg {
     using FMT = FMT_;
     static constexpr int BM = BM_, BN = BN_, BK = BK_, NP = NP_;
-    static constexpr bool PHASED = PHASED_, ILV = ILV_;
+    static constexpr bool PHASED = PHASED_, ILV = ILV_, STREAM = STREAM_, EARLY = EARLY_;
     static constexpr int WGM = WGM_, WGN = WGN_, NT = 64 * WGM_ * WGN_;
     static constexpr int WTM = BM / WGM, WTN = BN / WGN;
     static constexpr int FM = WTM / 16, FN = WTN / 16;
@@ -627,6 +639,8 @@ struct CoreCfg {
     static_assert(CHUNKS % NT == 0, "stage must split evenly over the threads");
     static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 32 == 0, "tile shape");
     static_assert(!ILV || (PHASED && STAGES == 2 && KSTEPS == 1), "interleaved staging needs phased reads, two stages");
+    static_assert(ILV == STREAM && (!STREAM || FM >= 4), "the interleaved staging runs with streamed reads");
+    static_assert(!EARLY || (STREAM && NP >= 2 && FN + 2 <= FM), "early reads: streamed, last product resident");
 };
 
 enum { FUSE_NONE = 0, FUSE_KCONT = 1, FUSE_RCONT = 2 };
@@ -737,6 +751,24 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
             for (int j = 0; j < 8; ++j) v[j] = (row < rows && k + j < K) ? p[j] : 0.0f;
         }
     };
+    // Plane copy, chunk c (A chunks first): global -> registers, and
+    // registers -> LDS buffer buf.
+    auto load_chunk = [&](int kt, int c) {
+        if (c < A_CH) {
+            const int e = c * CF::NT * 8, p = e / (BM * BK), off = e % (BM * BK);
+            pa[c] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + (long)kt * Mp * BK + off);
+        } else {
+            const int e = (c - A_CH) * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
+            pb[c - A_CH] = *reinterpret_cast<const V8 *>(b_src + p * b_plane + (long)kt * Np * BK + off);
+        }
+    };
+    auto store_chunk = [&](int buf, int c) {
+        E *a_reg = lds + buf * STAGE, *b_reg = a_reg + NP * CF::A_PLANE;
+        if (c < A_CH)
+            *reinterpret_cast<V8 *>(a_reg + t * 8 + c * CF::NT * 8) = pa[c];
+        else
+            *reinterpret_cast<V8 *>(b_reg + t * 8 + (c - A_CH) * CF::NT * 8) = pb[c - A_CH];
+    };
     auto load_stage = [&](int kt) {
         const int k0 = kt * BK;
         if constexpr (FA == FUSE_RCONT) {
@@ -748,12 +780,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                 load8(Af, lda, M, m0 + r, k0 + c * 8, vec_ab & 1, fa[i]);
             }
         } else {
-            const long a_k = (long)kt * Mp * BK;
 #pragma unroll
-            for (int i = 0; i < A_CH; ++i) {
-                const int e = i * CF::NT * 8, p = e / (BM * BK), off = e % (BM * BK);
-                pa[i] = *reinterpret_cast<const V8 *>(a_src + p * a_plane + a_k + off);
-            }
+            for (int i = 0; i < A_CH; ++i) load_chunk(kt, i);
         }
         if constexpr (FB == FUSE_RCONT) {
             load_rc(Bf, ldb, N, n0 + (t % (BN / 2)) * 2, k0 + (t / (BN / 2)) * 8, (vec_ab & 2) != 0, fb);
@@ -764,12 +792,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                 load8(Bf, ldb, N, n0 + r, k0 + c * 8, (vec_ab & 2) != 0, fb[i]);
             }
         } else {
-            const long b_k = (long)kt * Np * BK;
 #pragma unroll
-            for (int i = 0; i < B_CH; ++i) {
-                const int e = i * CF::NT * 8, p = e / (BN * BK), off = e % (BN * BK);
-                pb[i] = *reinterpret_cast<const V8 *>(b_src + p * b_plane + b_k + off);
-            }
+            for (int i = 0; i < B_CH; ++i) load_chunk(kt, A_CH + i);
         }
     };
     // Split one fused chunk (scaled formats: times 2^-e of its row) into its
@@ -806,7 +830,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < A_CH; ++i) *reinterpret_cast<V8 *>(a_reg + t * 8 + i * CF::NT * 8) = pa[i];
+            for (int i = 0; i < A_CH; ++i) store_chunk(buf, i);
         }
         if constexpr (FB == FUSE_RCONT) {
 #pragma unroll
@@ -822,7 +846,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < B_CH; ++i) *reinterpret_cast<V8 *>(b_reg + t * 8 + i * CF::NT * 8) = pb[i];
+            for (int i = 0; i < B_CH; ++i) store_chunk(buf, A_CH + i);
         }
     };
 
@@ -848,6 +872,78 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     __syncthreads();
 
     const int frow = lane & 15, fc = lane >> 4;
+    if constexpr (CF::EARLY && ILV) {
+        // Group (q, i): the FN MFMAs of product q with A fragment i.  In the
+        // products before the last, a group reads the A fragment of group
+        // (q, i + LA) and (last FN groups) a B fragment of product q + 1, and
+        // takes its share of the stage copy (K tile kt + 1 into the idle
+        // buffer, K tile kt + 2 into registers).  The barrier then opens the
+        // last product, whose planes are held in registers; its groups read
+        // K tile kt + 1's first B and A fragments from the other buffer.
+        // Every wave has finished reading a buffer (the products before the
+        // barrier) before any wave writes it again (after the next barrier).
+        constexpr int NQ = Products<NP>::N, LA = 2, NCH = A_CH + B_CH, NGC = (NQ - 1) * FM;
+        constexpr int a0 = Products<NP>::a[0], b0 = Products<NP>::b[0];
+        static_assert(!first_use_a<NP>(NQ - 1) && !first_use_b<NP>(NQ - 1), "last product from registers");
+        V8 fa[NP][FM], fb[NP][FN], na[LA], nb[FN];
+        auto frag = [&](const E *base, int row) {
+            return *reinterpret_cast<const V8 *>(base + row * BK + swz_chunk<CPR>(row, fc) * 8);
+        };
+        auto a_row = [&](int i) { return wm * CF::WTM + i * 16 + frow; };
+        auto b_row = [&](int j) { return wn * CF::WTN + j * 16 + frow; };
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[b0][j] = frag(lds + NP * CF::A_PLANE + b0 * CF::B_PLANE, b_row(j));
+#pragma unroll
+        for (int i = 0; i < LA; ++i) fa[a0][i] = frag(lds + a0 * CF::A_PLANE, a_row(i));
+        for (int kt = kt0; kt < ktiles; ++kt) {
+            const int buf = (kt - kt0) & 1;
+            const E *as = lds + buf * STAGE, *bs = as + NP * CF::A_PLANE;
+            const E *an = lds + (buf ^ 1) * STAGE, *bn = an + NP * CF::A_PLANE;
+            static_for<NQ>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int qa = Products<NP>::a[q], qb = Products<NP>::b[q];
+                constexpr bool last = q == NQ - 1;
+                constexpr int qn = last ? q : q + 1;
+                static_for<FM>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    __builtin_amdgcn_sched_barrier(0);
+                    if constexpr (!last) {
+                        if constexpr (i + LA < FM) {
+                            if constexpr (first_use_a<NP>(q)) fa[qa][i + LA] = frag(as + qa * CF::A_PLANE, a_row(i + LA));
+                        } else if constexpr (first_use_a<NP>(qn)) {
+                            constexpr int pn = Products<NP>::a[qn];
+                            fa[pn][i + LA - FM] = frag(as + pn * CF::A_PLANE, a_row(i + LA - FM));
+                        }
+                        if constexpr (first_use_b<NP>(qn) && i >= FM - FN) {
+                            constexpr int pn = Products<NP>::b[qn];
+                            fb[pn][i - (FM - FN)] = frag(bs + pn * CF::B_PLANE, b_row(i - (FM - FN)));
+                        }
+                        static_for<NCH>([&](auto cc) {
+                            constexpr int c = decltype(cc)::value;
+                            if constexpr (c * NGC / NCH == q * FM + i) {
+                                store_chunk(buf ^ 1, c);
+                                load_chunk(min(kt + 2, ktiles - 1), c);
+                            }
+                        });
+                    } else {
+                        if constexpr (i == 0) {
+                            __syncthreads();
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                        if constexpr (i < FN) nb[i] = frag(bn + b0 * CF::B_PLANE, b_row(i));
+                        else if constexpr (i - FN < LA) na[i - FN] = frag(an + a0 * CF::A_PLANE, a_row(i - FN));
+                    }
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[qa][i], fb[qb][j], acc[i][j]);
+                });
+            });
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fb[b0][j] = nb[j];
+#pragma unroll
+            for (int i = 0; i < LA; ++i) fa[a0][i] = na[i];
+        }
+    } else
     for (int kt = kt0; kt < ktiles; ++kt) {
         const int buf = (STAGES == 2) ? ((kt - kt0) & 1) : 0;
         const bool more = kt + 1 < ktiles;
@@ -857,6 +953,64 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
 #pragma unroll
         for (int s = 0; s < CF::KSTEPS; ++s) {
             V8 fa[NP][FM], fb[NP][FN];
+            if constexpr (CF::STREAM && ILV) {
+                // Group (q, i): the FN MFMAs of product q with A fragment i.
+                // Before its MFMAs a group reads the A fragment that group
+                // (q, i + LA) needs (product q + 1's first ones near the end
+                // of q), and each of the last FN groups of product q one B
+                // fragment of product q + 1.  Planes used again later stay in
+                // registers.  A scheduling barrier around every group keeps
+                // the reads in this order; the last product deals the stage
+                // copy out one chunk per group.
+                constexpr int NQ = Products<NP>::N, LA = 2, NCH = A_CH + B_CH;
+                auto read_a = [&](int p, int i) {
+                    const int row = wm * CF::WTM + i * 16 + frow;
+                    fa[p][i] = *reinterpret_cast<const V8 *>(as + p * CF::A_PLANE + row * BK +
+                                                             swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                };
+                auto read_b = [&](int p, int j) {
+                    const int row = wn * CF::WTN + j * 16 + frow;
+                    fb[p][j] = *reinterpret_cast<const V8 *>(bs + p * CF::B_PLANE + row * BK +
+                                                             swz_chunk<CPR>(row, s * 4 + fc) * 8);
+                };
+#pragma unroll
+                for (int j = 0; j < FN; ++j) read_b(Products<NP>::b[0], j);
+#pragma unroll
+                for (int i = 0; i < LA; ++i) read_a(Products<NP>::a[0], i);
+                static_for<NQ>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int qa = Products<NP>::a[q], qb = Products<NP>::b[q];
+                    constexpr bool last = q == NQ - 1;
+                    constexpr int qn = last ? q : q + 1;
+                    static_for<FM>([&](auto ic) {
+                        constexpr int i = decltype(ic)::value;
+                        __builtin_amdgcn_sched_barrier(0);
+                        if constexpr (i + LA < FM) {
+                            if constexpr (first_use_a<NP>(q)) read_a(qa, i + LA);
+                        } else if constexpr (!last && first_use_a<NP>(qn)) {
+                            read_a(Products<NP>::a[qn], i + LA - FM);
+                        }
+                        if constexpr (!last && first_use_b<NP>(qn) && i >= FM - FN)
+                            read_b(Products<NP>::b[qn], i - (FM - FN));
+                        if constexpr (last) {
+                            // Chunks c with c * FM / NCH == i: store into the
+                            // idle buffer, then load stage kt + 2 (the last
+                            // steps re-load the last stage: addresses in range).
+                            static_for<NCH>([&](auto cc) {
+                                constexpr int c = decltype(cc)::value;
+                                if constexpr (c * FM / NCH == i) {
+                                    store_chunk(buf ^ 1, c);
+                                    load_chunk(min(kt + 2, ktiles - 1), c);
+                                }
+                            });
+                        }
+#pragma unroll
+                        for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[qa][i], fb[qb][j], acc[i][j]);
+                    });
+                });
+                __builtin_amdgcn_sched_barrier(0);
+                continue;
+            }
             if constexpr (CF::PHASED) {
                 static_for<Products<NP>::N>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
@@ -879,30 +1033,10 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                                                                       swz_chunk<CPR>(row, s * 4 + fc) * 8);
                         }
                     }
-                    if constexpr (ILV && q == Products<NP>::N - 1) {
-                        // Last product: its own scheduling region, in which the
-                        // stage copy is dealt between the MFMAs.  No branch may
-                        // split the region, so the last step stores into the idle
-                        // buffer and re-loads the last stage (addresses in range).
-                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int i = 0; i < FM; ++i)
+                    for (int i = 0; i < FM; ++i)
 #pragma unroll
-                            for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[pa][i], fb[pb][j], acc[i][j]);
-                        store_stage(buf ^ 1);
-                        load_stage(min(kt + 2, ktiles - 1));
-                        constexpr int NCH = A_CH + B_CH, PER = (FM * FN) / NCH > 0 ? (FM * FN) / NCH : 1;
-                        static_for<NCH>([&](auto) {
-                            __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);   // MFMA
-                            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);     // DS write
-                            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
-                        });
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < FM; ++i)
-#pragma unroll
-                            for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[pa][i], fb[pb][j], acc[i][j]);
-                    }
+                        for (int j = 0; j < FN; ++j) acc[i][j] = FMT::mfma(fa[pa][i], fb[pb][j], acc[i][j]);
                 });
                 continue;
             }
@@ -988,10 +1122,10 @@ template <int NP, class FMT = FmtBf16>
 struct Table;
 // V0-V3: 4 waves (2 x 2), one per SIMD.  V4, V5: 8 or 16 waves, 2 or 4 per
 // SIMD, which hide each other's LDS and global latency (measured at 8192^3:
-// 15-20% faster core than V0).  V6: 8 waves with phased reads and the
-// interleaved stage copy (ILV); 5-7% faster than V5 for two planes.  NP = 1, 2: 256x256x32 tiles (32 / 64 KiB
-// per stage); NP = 3: 256x128x32 (72 KiB per stage).  Smaller tiles serve
-// small grids.
+// 15-20% faster core than V0).  V6: 8 waves with streamed reads, the
+// interleaved stage copy and (NP >= 2) the early barrier; 6-10% faster than
+// V5 at 8192^3.  NP = 1, 2: 256x256x32 tiles (32 / 64 KiB per stage); NP = 3:
+// 256x128x32 (72 KiB per stage).  Smaller tiles serve small grids.
 template <>
 struct Table<1> {
     using V0 = CoreCfg<256, 256, 32, 1>;
@@ -1000,7 +1134,7 @@ struct Table<1> {
     using V3 = CoreCfg<64, 64, 32, 1>;
     using V4 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 1, FmtBf16, 4, 4, true>;
-    using V6 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, true, true>;
+    using V6 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, true, true, true>;
 };
 template <>
 struct Table<2> {
@@ -1010,7 +1144,7 @@ struct Table<2> {
     using V3 = CoreCfg<64, 64, 32, 2>;
     using V4 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 2, FmtBf16, 4, 4, true>;
-    using V6 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, true, true>;
+    using V6 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, true, true, true, true>;
 };
 template <>
 struct Table<3> {
@@ -1020,7 +1154,7 @@ struct Table<3> {
     using V3 = CoreCfg<64, 64, 32, 3>;
     using V4 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true>;
     using V5 = CoreCfg<256, 128, 32, 3, FmtBf16, 4, 2, true>;
-    using V6 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true, true>;
+    using V6 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true, true, true, true>;
 };
 template <>
 struct Table<2, FmtF16> {
@@ -1030,7 +1164,7 @@ struct Table<2, FmtF16> {
     using V3 = CoreCfg<64, 64, 32, 2, FmtF16>;
     using V4 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 2, FmtF16, 4, 4, true>;
-    using V6 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, true, true>;
+    using V6 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, true, true, true, true>;
 };
 constexpr int kNumVariants = 7;
 
@@ -1041,7 +1175,7 @@ const char *cfg_name() {
         char buf[48];
         snprintf(buf, sizeof buf, "%sx%d_nt_%dx%dx%d_w%dx%d%s", CF::FMT::name,
                  CF::NP == 1 ? 1 : (CF::NP == 2 ? 3 : 6), CF::BM, CF::BN, CF::BK, CF::WGM, CF::WGN,
-                 CF::ILV ? "_ph_ilv" : (CF::PHASED ? "_ph" : ""));
+                 CF::EARLY ? "_st_early" : (CF::STREAM ? "_st" : (CF::PHASED ? "_ph" : "")));
         return std::string(buf);
     }();
     return name.c_str();
@@ -1366,12 +1500,12 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
     const long tiles_big = ((g.M + big_m - 1) / big_m) * ((g.N + big_n - 1) / big_n);
     if (tiles_big >= cus) {
         // A skinny side whose operand (either layout) the 8-wave variant
-        // splits in the core (no preparation pass): V4.  Otherwise two planes run fastest on
-        // the 8-wave interleaved-staging V6, one plane and three planes on
-        // the 16-wave V5.
+        // splits in the core (no preparation pass): V4.  Otherwise two and
+        // three planes run fastest on the streamed V6, one plane on the
+        // 16-wave V5.
         const long tm = (g.M + big_m - 1) / big_m, tn = (g.N + big_n - 1) / big_n;
         const bool fuse = np <= 2 && (tn <= 2 || tm <= 2);
-        return fuse ? 4 : (np == 2 ? 6 : 5);
+        return fuse ? 4 : (np >= 2 ? 6 : 5);
     }
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= cus / 2) return 1;

@@ -160,11 +160,19 @@ int fp32_pick_variant(const Gemm<float> &g) {
     // bandwidth that bounds the 8x8 kernel (measured 6-8% faster at 8192^3).
     // Else enough 128x128 tiles: those; otherwise shrink the tile so that the
     // grid still covers the chip.
+    // Row-major NN (A k-contiguous, B row-contiguous) runs best on the 16x8
+    // micro-tile: 98-100 against 92-93 TF/s for 8x16 at 4096^3 and 8192^3,
+    // interleaved on one box; NT and TN stay on 8x16 (within 1-3% of 8x8).
+    // A grid of 128x256 tiles covering half the CUs still takes 8x16, the
+    // launcher splitting K (2048^3: 77 against 74 TF/s on 64x64 tiles).
     const int cus = device_cu_count();
     const long t128x256 = (long)((g.M + 127) / 128) * ((g.N + 255) / 256);
+    const long t256x128 = (long)((g.M + 255) / 256) * ((g.N + 127) / 128);
+    if (!g.ta && !g.tb && t256x128 >= 2L * cus) return 4;
     if (t128x256 >= 2L * cus) return 5;
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
     if (t128 >= 2L * cus) return 0;
+    if (2 * t128x256 >= cus && g.K >= 1024 && g.splits != 1) return 5;
     const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
     if (g.M >= 4 * g.N && t64 >= cus) return 2;
     if (g.N >= 4 * g.M && t64 >= cus) return 3;
