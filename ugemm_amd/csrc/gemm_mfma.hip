@@ -583,15 +583,6 @@ constexpr bool first_use_b(int q) {
     return true;
 }
 
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // WGM x WGN waves per workgroup.  PHASED: fragments of each plane are read
 // just before the first product that needs them, behind a scheduling
 // barrier, so that at most three planes' fragments are live (2 waves per
@@ -1070,46 +1061,56 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
 
     // Epilogue: C = alpha * acc (+ beta * C), rows 4*(l/16)+r, column l%16 of each fragment;
     // scaled formats first undo the row and column scales: acc * 2^(e_m + e_n).
-    // The C values of JB fragments are all loaded before any is stored: a
-    // load cannot move above a store to C that might alias it, so
-    // interleaving them would expose one memory latency per element.  With
-    // 256 accumulators per lane there is no room for the batch (it spills):
-    // those variants interleave.
+    // C is read in batches of IB x JB fragments (64 values per lane), all
+    // loaded before any is stored: a load cannot move above a store to C
+    // that might alias it, so every batch exposes one memory latency.  With
+    // 256 accumulators per lane there is no room for a batch (it spills),
+    // nor with 16 waves: those variants interleave.
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
     const bool has_epi = !GR && !part && epi.active();
     constexpr bool BATCH = FM * FN * 4 < 256;
     constexpr int JB = !BATCH ? 1 : (CF::NT >= 1024 ? 1 : FN);
+    constexpr int IB = (JB < FN) ? 1 : (64 / (4 * FN) < 1 ? 1 : (64 / (4 * FN) > FM ? FM : 64 / (4 * FN)));
+    static_assert(FM % IB == 0, "whole batches of fragment rows");
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i0 = 0; i0 < FM; i0 += IB)
 #pragma unroll
     for (int j0 = 0; j0 < FN; j0 += JB) {
-        const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
-        float cv[JB][4];
+        float cv[IB][JB][4];
         if (BATCH && be != 0.0f) {
 #pragma unroll
-            for (int jj = 0; jj < JB; ++jj) {
-                const int n = n0 + wn * CF::WTN + (j0 + jj) * 16 + (lane & 15);
+            for (int ii = 0; ii < IB; ++ii) {
+                const int mb = m0 + wm * CF::WTM + (i0 + ii) * 16 + (lane >> 4) * 4;
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    cv[jj][r] = (interior || (mb + r < M && n < N)) ? Co[(long)(mb + r) * ldo + n] : 0.0f;
+                for (int jj = 0; jj < JB; ++jj) {
+                    const int n = n0 + wn * CF::WTN + (j0 + jj) * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[ii][jj][r] = (interior || (mb + r < M && n < N)) ? Co[(long)(mb + r) * ldo + n] : 0.0f;
+                }
             }
         }
 #pragma unroll
-        for (int jj = 0; jj < JB; ++jj) {
-            const int j = j0 + jj;
-            const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
-            int en = 0;
-            if constexpr (FMT::SCALED) en = row_exp(amax_b[n]);   // n < Np: inside the array
+        for (int ii = 0; ii < IB; ++ii) {
+            const int i = i0 + ii;
+            const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = mb + r;
-                if (interior || (m < M && n < N)) {
-                    float a = acc[i][j][r];
-                    if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
-                    float *cp = Co + (long)m * ldo + n;
-                    float v = (be == 0.0f) ? al * a : fmaf(al, a, be * (BATCH ? cv[jj][r] : *cp));
-                    if (has_epi) v = apply_epilogue(epi, v, m, n);
-                    *cp = v;
+            for (int jj = 0; jj < JB; ++jj) {
+                const int j = j0 + jj;
+                const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+                int en = 0;
+                if constexpr (FMT::SCALED) en = row_exp(amax_b[n]);   // n < Np: inside the array
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mb + r;
+                    if (interior || (m < M && n < N)) {
+                        float a = acc[i][j][r];
+                        if constexpr (FMT::SCALED) a = ldexpf(a, row_exp(amax_a[m]) + en);
+                        float *cp = Co + (long)m * ldo + n;
+                        float v = (be == 0.0f) ? al * a : fmaf(al, a, be * (BATCH ? cv[ii][jj][r] : *cp));
+                        if (has_epi) v = apply_epilogue(epi, v, m, n);
+                        *cp = v;
+                    }
                 }
             }
         }

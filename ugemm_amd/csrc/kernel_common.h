@@ -2,8 +2,20 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <utility>
 
 namespace ugemm {
+
+// static_for<N>(f): f(std::integral_constant<int, i>{}) for i = 0 .. N-1,
+// unrolled at compile time (indices usable in if constexpr and templates).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // Maps a launch's linear block id to a (tile_m, tile_n) pair of C tiles.
 // 1) XCD de-interleave: hardware hands consecutive block ids round-robin to

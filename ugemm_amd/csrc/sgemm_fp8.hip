@@ -223,13 +223,13 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
     const unsigned char *b_src = Bq + (long)n0 * BK + t * 16;
     // A native vector type (not HIP's uint4 struct), so that the array stays in registers.
     u32x4 regs[PER_T];
+    auto load_chunk = [&](int kt, int i) {
+        if (i < A_CH) regs[i] = *reinterpret_cast<const u32x4 *>(a_src + (long)kt * Mp * BK + i * CF::NT * 16);
+        else regs[i] = *reinterpret_cast<const u32x4 *>(b_src + (long)kt * Np * BK + (i - A_CH) * CF::NT * 16);
+    };
     auto load_stage = [&](int kt) {
-        const long ak = (long)kt * Mp * BK, bk = (long)kt * Np * BK;
 #pragma unroll
-        for (int i = 0; i < PER_T; ++i) {
-            if (i < A_CH) regs[i] = *reinterpret_cast<const u32x4 *>(a_src + ak + i * CF::NT * 16);
-            else regs[i] = *reinterpret_cast<const u32x4 *>(b_src + bk + (i - A_CH) * CF::NT * 16);
-        }
+        for (int i = 0; i < PER_T; ++i) load_chunk(kt, i);
     };
     auto store_stage = [&](int buf) {
         unsigned char *base = smem8 + buf * CF::STAGE + t * 16;
@@ -256,46 +256,58 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
         const u32x4 hi = *reinterpret_cast<const u32x4 *>(base + row * BK + (((2 * fq + 1) ^ fp8_swz(row)) << 4));
         return int8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
     };
+    // The scales of K tile kt + 1 are loaded at the top of step kt, before
+    // that step's stage loads: the loads complete in order, so waiting for
+    // a scale never waits for a stage load issued after it.  (Loaded in the
+    // step that uses them, they made every step wait for all of the stage
+    // loads of the step before.)
+    const int *__restrict__ ea_w = ea + (long)a_blk * kblocks;
+    const int *__restrict__ eb_w = eb + (long)b_blk * kblocks;
+    int sa_next = ea_w[0], sb_next = eb_w[0];
     for (int kt = 0; kt < kblocks; ++kt) {
         const int buf = kt & 1;
         const bool more = kt + 1 < kblocks;
+        const int sa = sa_next + 127, sb = sb_next + 127;
+        sa_next = ea_w[min(kt + 1, kblocks - 1)];
+        sb_next = eb_w[min(kt + 1, kblocks - 1)];
         if (!CF::ILV && more) load_stage(kt + 1);
-        const int sa = ea[(long)a_blk * kblocks + kt] + 127, sb = eb[(long)b_blk * kblocks + kt] + 127;
         const unsigned char *as = smem8 + buf * CF::STAGE;
         const unsigned char *bs = as + BM * BK;
         int8v fb[FN];
 #pragma unroll
         for (int j = 0; j < FN; ++j) fb[j] = frag(bs, wn * CF::WTN + j * 16 + frow);
         if constexpr (CF::ILV) {
-            // Row-block i's A fragment is read while row-block i-1 multiplies.
-            int8v fa_cur = frag(as, wm * CF::WTM + frow), fa_next;
+            // Group i: the FN MFMAs of row-block i.  Before its MFMAs a group
+            // reads the A fragment of row-block i + 2, and each group of the
+            // second half takes its share of the stage copy (stage kt + 1
+            // from registers into the idle buffer, then the loads of stage
+            // kt + 2; branch-free: the last step stores into the idle buffer
+            // and re-loads the last stage).  A scheduling barrier around each
+            // group keeps this order.
+            constexpr int LA = 2, H = FM / 2;
+            int8v fa[FM];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                if (i + 1 < FM) fa_next = frag(as, wm * CF::WTM + (i + 1) * 16 + frow);
+            for (int i = 0; i < LA; ++i) fa[i] = frag(as, wm * CF::WTM + i * 16 + frow);
+            static_for<FM>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                __builtin_amdgcn_sched_barrier(0);
+                if constexpr (i + LA < FM) fa[i + LA] = frag(as, wm * CF::WTM + (i + LA) * 16 + frow);
+                if constexpr (i >= H) {
+                    static_for<PER_T>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value;
+                        if constexpr (c * H / PER_T == i - H) {
+                            *reinterpret_cast<u32x4 *>(smem8 + (buf ^ 1) * CF::STAGE + t * 16 + c * CF::NT * 16) =
+                                regs[c];
+                            load_chunk(min(kt + 2, kblocks - 1), c);
+                        }
+                    });
+                }
 #pragma unroll
                 for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa_cur, fb[j], acc[i][j], 0, 0, 0,
-                                                                                 sa, 0, sb);
-                if (i + 1 == FM / 2) {
-                    // The second half's scheduling region also carries the
-                    // stage copy: stage kt+1 from registers to the idle buffer,
-                    // then the loads of stage kt+2 (branch-free: the last step
-                    // stores into the idle buffer and re-loads the last stage).
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                if (i + 1 == FM) {
-                    store_stage(buf ^ 1);
-                    load_stage(min(kt + 2, kblocks - 1));
-                    constexpr int PER = (FM / 2) * FN / PER_T > 0 ? (FM / 2) * FN / PER_T : 1;
-#pragma unroll
-                    for (int c = 0; c < PER_T; ++c) {
-                        __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);   // MFMA
-                        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);     // DS write
-                        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
-                    }
-                }
-                fa_cur = fa_next;
-            }
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, sa,
+                                                                                 0, sb);
+            });
+            __builtin_amdgcn_sched_barrier(0);
         } else {
             int8v fa[FM];
 #pragma unroll
@@ -311,34 +323,45 @@ __global__ __launch_bounds__(CF::NT) void fp8_scaled_core(int M, int N, int Kp, 
         __syncthreads();
     }
 
-    // C values of one fragment row are loaded before any is stored (a load
-    // cannot pass a possibly aliasing store: one latency per row, not per
-    // element) -- unless 256 accumulators per lane leave no room (it spills).
+    // C is read in batches of IB fragment rows (64 values per lane), all
+    // loaded before any is stored (a load cannot pass a possibly aliasing
+    // store: one latency per batch, not per element) -- unless 256
+    // accumulators per lane leave no room (it spills).
     const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
     constexpr bool BATCH = FM * FN * 4 < 256;
+    constexpr int IB = !BATCH ? 1 : (64 / (4 * FN) > FM ? FM : (64 / (4 * FN) < 1 ? 1 : 64 / (4 * FN)));
+    static_assert(FM % IB == 0, "whole batches of fragment rows");
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
-        float cv[FN][4];
+    for (int i0 = 0; i0 < FM; i0 += IB) {
+        float cv[IB][FN][4];
         if (BATCH && beta != 0.0f) {
+#pragma unroll
+            for (int ii = 0; ii < IB; ++ii) {
+                const int mb = m0 + wm * CF::WTM + (i0 + ii) * 16 + (lane >> 4) * 4;
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        cv[ii][j][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int ii = 0; ii < IB; ++ii) {
+            const int i = i0 + ii;
+            const int mb = m0 + wm * CF::WTM + i * 16 + (lane >> 4) * 4;
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
                 const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    cv[j][r] = (interior || (mb + r < M && n < N)) ? C[(long)(mb + r) * ldc + n] : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int n = n0 + wn * CF::WTN + j * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = mb + r;
-                if (interior || (m < M && n < N)) {
-                    float *cp = C + (long)m * ldc + n;
-                    if (beta == 0.0f) *cp = alpha * acc[i][j][r];
-                    else *cp = fmaf(alpha, acc[i][j][r], beta * (BATCH ? cv[j][r] : *cp));
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mb + r;
+                    if (interior || (m < M && n < N)) {
+                        float *cp = C + (long)m * ldc + n;
+                        if (beta == 0.0f) *cp = alpha * acc[i][j][r];
+                        else *cp = fmaf(alpha, acc[i][j][r], beta * (BATCH ? cv[ii][j][r] : *cp));
+                    }
                 }
             }
         }
