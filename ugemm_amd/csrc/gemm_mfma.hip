@@ -340,10 +340,12 @@ __global__ __launch_bounds__(256) void split_prep_rcont(const float *__restrict_
 }
 
 // block_job[b]: the job of preparation block b (one load, where a search of
-// the job table took ~10 dependent loads per block).  A k-contiguous job
-// takes a 64-row x 128-k block per block (prep_kcont_tile: four 512-byte row
-// segments per load instruction, 256-byte runs per store), nbx blocks along
-// the rows; a row-contiguous job a 64 x 64 transposing tile.
+// the job table took ~10 dependent loads per block).  A block prepares
+// kPrepSpan consecutive k tiles of one 64-row stripe (fewer job lookups and
+// block starts per byte): 64-row x 128-k tiles of a k-contiguous job
+// (prep_kcont_tile: four 512-byte row segments per load instruction,
+// 256-byte runs per store), 64 x 64 transposing tiles of a row-contiguous
+// one; nbx blocks along the rows.
 template <int NP, class FMT>
 __global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restrict__ jobs,
                                                           const int *__restrict__ block_job) {
@@ -351,12 +353,17 @@ __global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restr
     const PrepJob j = jobs[block_job[blockIdx.x]];
     const int lb = (int)blockIdx.x - j.block0;
     typename FMT::E *out = static_cast<typename FMT::E *>(j.out);
+    const int r0 = (lb % j.nbx) * 64, kb = (lb / j.nbx) * kPrepSpan;
     if (j.kcont) {
-        prep_kcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
-                                 j.amax, (lb % j.nbx) * 64, (lb / j.nbx) * 128);
+        for (int s = 0; s < kPrepSpan && (kb + s) * 128 < j.k_pad; ++s)
+            prep_kcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
+                                     j.amax, r0, (kb + s) * 128);
     } else {
-        prep_rcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
-                                 j.amax, (lb % j.nbx) * 64, (lb / j.nbx) * 64, tile);
+        for (int s = 0; s < kPrepSpan && (kb + s) * 64 < j.k_pad; ++s) {
+            if (s > 0) __syncthreads();   // the previous tile's reads of the LDS tile are done
+            prep_rcont_tile<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad, j.k_pad,
+                                     j.amax, r0, (kb + s) * 64, tile);
+        }
     }
 }
 

@@ -481,7 +481,8 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
         j.block0 = (int)blocks;
         j.nbx = rows_pad / 64;
-        blocks += (long)(rows_pad / 64) * (kcont ? (k_pad + 127) / 128 : k_pad / 64);
+        const long ktiles = kcont ? (k_pad + 127) / 128 : k_pad / 64;
+        blocks += (long)(rows_pad / 64) * ((ktiles + kPrepSpan - 1) / kPrepSpan);
         jobs.push_back(j);
     };
     for (int i = 0; i < (int)p->probs.size(); ++i) {

@@ -60,8 +60,10 @@ struct GroupedSplitTile {
 };
 // One operand of a grouped split-form plan to prepare: its FP32 source and
 // its planes, and the first block of the job in the preparation grid.
-// k-contiguous jobs take a 64-row x 128-k block per block, row-contiguous
-// jobs a 64 x 64 tile per block (nbx blocks along the rows either way).
+// A block takes kPrepSpan consecutive k tiles of one 64-row stripe: 64 x 128
+// tiles of a k-contiguous job, 64 x 64 tiles of a row-contiguous one (nbx
+// blocks along the rows either way).
+constexpr int kPrepSpan = 4;
 struct PrepJob {
     const float *X;
     long ld;
