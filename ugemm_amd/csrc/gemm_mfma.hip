@@ -723,7 +723,19 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     float fa[FA ? FA_CH : 1][8], fb[FB ? FB_CH : 1][8];
     // Row-contiguous source: rows [r, r+2) of k-lines k .. k+7 into v[i][j]
     // (row r+i, k+j); zeros outside rows x K.  vec: 8-byte loads allowed.
-    auto load_rc = [&](const float *X, long ld, int rows, int r, int k, bool vec, float (&v)[2][8]) {
+    // full: the whole stage lies inside rows x K and vector loads are
+    // allowed (uniform over the block): unguarded loads, no per-element
+    // branches.
+    auto load_rc = [&](const float *X, long ld, int rows, int r, int k, bool vec, float (&v)[2][8], bool full) {
+        if (full) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float2 x = *reinterpret_cast<const float2 *>(X + (long)(k + j) * ld + r);
+                v[0][j] = x.x;
+                v[1][j] = x.y;
+            }
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const float *p = X + (long)(k + j) * ld + r;
@@ -738,9 +750,9 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
         }
     };
     // 8 floats of row `row`, k in [k, k+8), zeros outside rows x K.
-    auto load8 = [&](const float *X, long ld, int rows, int row, int k, bool vec, float (&v)[8]) {
+    auto load8 = [&](const float *X, long ld, int rows, int row, int k, bool vec, float (&v)[8], bool full) {
         const float *p = X + (long)row * ld + k;
-        if (row < rows && k + 8 <= K && vec) {
+        if (full || (row < rows && k + 8 <= K && vec)) {
             const float4 x0 = *reinterpret_cast<const float4 *>(p);
             const float4 x1 = *reinterpret_cast<const float4 *>(p + 4);
             v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w; v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
@@ -769,25 +781,30 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     };
     auto load_stage = [&](int kt) {
         const int k0 = kt * BK;
+        const bool kin = k0 + BK <= K;
         if constexpr (FA == FUSE_RCONT) {
-            load_rc(Af, lda, M, m0 + (t % (BM / 2)) * 2, k0 + (t / (BM / 2)) * 8, vec_ab & 1, fa);
+            load_rc(Af, lda, M, m0 + (t % (BM / 2)) * 2, k0 + (t / (BM / 2)) * 8, vec_ab & 1, fa,
+                    kin && (vec_ab & 1) && m0 + BM <= M);
         } else if constexpr (FA == FUSE_KCONT) {
+            const bool full = kin && (vec_ab & 1) && m0 + BM <= M;
 #pragma unroll
             for (int i = 0; i < FA_CH; ++i) {
                 const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
-                load8(Af, lda, M, m0 + r, k0 + c * 8, vec_ab & 1, fa[i]);
+                load8(Af, lda, M, m0 + r, k0 + c * 8, vec_ab & 1, fa[i], full);
             }
         } else {
 #pragma unroll
             for (int i = 0; i < A_CH; ++i) load_chunk(kt, i);
         }
         if constexpr (FB == FUSE_RCONT) {
-            load_rc(Bf, ldb, N, n0 + (t % (BN / 2)) * 2, k0 + (t / (BN / 2)) * 8, (vec_ab & 2) != 0, fb);
+            load_rc(Bf, ldb, N, n0 + (t % (BN / 2)) * 2, k0 + (t / (BN / 2)) * 8, (vec_ab & 2) != 0, fb,
+                    kin && (vec_ab & 2) && n0 + BN <= N);
         } else if constexpr (FB == FUSE_KCONT) {
+            const bool full = kin && (vec_ab & 2) && n0 + BN <= N;
 #pragma unroll
             for (int i = 0; i < FB_CH; ++i) {
                 const int f = t + CF::NT * i, r = f >> 2, c = (f & 3) ^ ((r >> 1) & 3);
-                load8(Bf, ldb, N, n0 + r, k0 + c * 8, (vec_ab & 2) != 0, fb[i]);
+                load8(Bf, ldb, N, n0 + r, k0 + c * 8, (vec_ab & 2) != 0, fb[i], full);
             }
         } else {
 #pragma unroll
@@ -864,9 +881,17 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     const long ldo = part ? (long)N : ldc;
     const float al = part ? 1.0f : alpha, be = part ? 0.0f : beta;
     constexpr bool ILV = CF::ILV && !FA && !FB;
+    // Fused operands on two stages (ESTORE): stage kt+1, loaded during step
+    // kt-1, is split into the idle buffer at the top of step kt and stage
+    // kt+2 is loaded right after, so the FP32 loads get a whole step of
+    // latency and the split of one block overlaps the MFMAs of others
+    // (split at the end of the step, every wave waited on its loads and then
+    // split in front of the barrier).  Branch-free like ILV: the last steps
+    // store into the idle buffer and re-load the last stage.
+    constexpr bool ESTORE = !ILV && (FA || FB) && STAGES == 2;
     load_stage(kt0);
     store_stage(0);
-    if constexpr (ILV) load_stage(min(kt0 + 1, ktiles - 1));
+    if constexpr (ILV || ESTORE) load_stage(min(kt0 + 1, ktiles - 1));
     __syncthreads();
 
     const int frow = lane & 15, fc = lane >> 4;
@@ -945,7 +970,12 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     for (int kt = kt0; kt < ktiles; ++kt) {
         const int buf = (STAGES == 2) ? ((kt - kt0) & 1) : 0;
         const bool more = kt + 1 < ktiles;
-        if (!ILV && more) load_stage(kt + 1);
+        if constexpr (ESTORE) {
+            store_stage(buf ^ 1);
+            load_stage(min(kt + 2, ktiles - 1));
+        } else if (!ILV && more) {
+            load_stage(kt + 1);
+        }
         const E *as = lds + buf * STAGE;
         const E *bs = as + NP * CF::A_PLANE;
 #pragma unroll
@@ -1062,7 +1092,7 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
                         acc[i][j] = FMT::mfma(fa[Products<NP>::a[q]][i], fb[Products<NP>::b[q]][j], acc[i][j]);
         }
         if (STAGES == 1) __syncthreads();
-        if (!ILV && more) store_stage(STAGES == 2 ? (buf ^ 1) : 0);
+        if (!ILV && !ESTORE && more) store_stage(STAGES == 2 ? (buf ^ 1) : 0);
         __syncthreads();
     }
 
