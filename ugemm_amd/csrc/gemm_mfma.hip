@@ -186,6 +186,13 @@ __device__ __forceinline__ void prep_kcont_tile(const float *__restrict__ X, lon
                                                 typename FMT::E *__restrict__ out, long plane_stride, int rows_pad,
                                                 int k_pad, const unsigned *__restrict__ amax, int r0, int k0) {
     const int t = threadIdx.x;
+    // Scaled formats: the row maxima are loaded first, so that their latency
+    // overlaps the operand loads instead of following them.
+    unsigned am[4] = {0u, 0u, 0u, 0u};
+    if constexpr (FMT::SCALED) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) am[i] = amax[r0 + ((t + 256 * i) >> 4)];
+    }
     float x[4][8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -206,7 +213,7 @@ __device__ __forceinline__ void prep_kcont_tile(const float *__restrict__ X, lon
         const int f = t + 256 * i, r = r0 + (f >> 4), k = k0 + (f & 15) * 8;
         if (k >= k_pad) continue;
         if constexpr (FMT::SCALED) {
-            const int e = -row_exp(amax[r]);
+            const int e = -row_exp(am[i]);
 #pragma unroll
             for (int j = 0; j < 8; ++j) x[i][j] = ldexpf(x[i][j], e);
         }
@@ -291,6 +298,15 @@ __device__ __forceinline__ void prep_rcont_tile(const float *__restrict__ X, lon
                                                 int k_pad, const unsigned *__restrict__ amax, int r0, int k0,
                                                 float (*tile)[65]) {
     const int t = threadIdx.x;
+    // Scaled formats: this thread's two row maxima, loaded before the tile.
+    unsigned am[2] = {0u, 0u};
+    if constexpr (FMT::SCALED) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = r0 + (t + i * 256) / 8;
+            am[i] = r < rows_pad ? amax[r] : 0u;
+        }
+    }
     // Read: 64 k-lines of 64 r each, as 16 float4 per line (coalesced along r).
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -321,7 +337,7 @@ __device__ __forceinline__ void prep_rcont_tile(const float *__restrict__ X, lon
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = tile[kk + j][rr];
         if constexpr (FMT::SCALED) {
-            const int e = -row_exp(amax[r]);
+            const int e = -row_exp(am[i]);
 #pragma unroll
             for (int j = 0; j < 8; ++j) x[j] = ldexpf(x[j], e);
         }
