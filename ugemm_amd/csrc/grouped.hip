@@ -273,21 +273,21 @@ struct ugemm_grouped_plan {
 namespace {
 
 void release_schedule(ugemm_grouped_plan *p) {
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     p->exec = nullptr;
     p->graph = nullptr;
-    hipFree(p->d_tiles);
-    hipFree(p->d_queues);
-    hipFree(p->d_heads);
-    hipFree(p->d_planes);
-    hipFree(p->d_jobs);
-    hipFree(p->d_sprobs);
-    hipFree(p->d_stiles);
-    hipFree(p->d_amax);
-    hipFree(p->d_ajobs);
-    hipFree(p->d_block_job);
-    hipFree(p->d_ablock_job);
+    (void)hipFree(p->d_tiles);
+    (void)hipFree(p->d_queues);
+    (void)hipFree(p->d_heads);
+    (void)hipFree(p->d_planes);
+    (void)hipFree(p->d_jobs);
+    (void)hipFree(p->d_sprobs);
+    (void)hipFree(p->d_stiles);
+    (void)hipFree(p->d_amax);
+    (void)hipFree(p->d_ajobs);
+    (void)hipFree(p->d_block_job);
+    (void)hipFree(p->d_ablock_job);
     p->d_amax = nullptr;
     p->d_ajobs = nullptr;
     p->d_block_job = nullptr;
@@ -389,7 +389,7 @@ int capture_plan(ugemm_grouped_plan *p) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -795,11 +795,11 @@ int ugemm_hip_grouped_run(ugemm_grouped_plan_t *p, void *stream) {
 
 void ugemm_hip_grouped_plan_destroy(ugemm_grouped_plan_t *p) {
     if (!p) return;
-    if (p->cap_stream) hipStreamSynchronize(p->cap_stream);
+    if (p->cap_stream) (void)hipStreamSynchronize(p->cap_stream);
     release_schedule(p);
-    hipFree(p->d_probs);
-    hipFree(p->d_ktab);
-    if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+    (void)hipFree(p->d_probs);
+    (void)hipFree(p->d_ktab);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     delete p;
 }
 
@@ -886,15 +886,15 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
         }
         if (rc) break;
         launch_plan_kernel(p, s);   // warm-up
-        hipEventRecord(e0, s);
+        (void)hipEventRecord(e0, s);
         for (int r = 0; r < 3; ++r) launch_plan_kernel(p, s);
-        hipEventRecord(e1, s);
+        (void)hipEventRecord(e1, s);
         if (hipEventSynchronize(e1) != hipSuccess) {
             rc = fail(UGEMM_ERR_HIP, "grouped tune: timing");
             break;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         if (best < 0 || ms < best_ms) {
             best = v;
             best_ms = ms;
@@ -908,11 +908,11 @@ int ugemm_hip_tune_grouped(ugemm_grouped_plan_t *p) {
         if (rc_run) rc = rc_run;
     }
     if (!rc) rc = p->split ? build_split_schedule(p, best) : build_schedule(p, best);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipStreamSynchronize(s);
-    hipStreamDestroy(s);
-    if (c_save) hipFree(c_save);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    if (c_save) (void)hipFree(c_save);
     if (rc) return rc;
     tune_record(TUNE_KEY_GROUPED, p->conv, p->mode, (int)p->probs.size(),
                 (int)std::min(p->flops / 1e6, 2e9), 0, best, best_ms / 3);

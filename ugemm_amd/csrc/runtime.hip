@@ -386,10 +386,10 @@ static int run_host(const Gemm<T> &h, int mode) {
         const long Mb = (long)round_up((size_t)((h.M + nb - 1) / nb), 256);
         std::vector<hipEvent_t> evA, evC;
         auto cleanup = [&]() {
-            hipStreamSynchronize(ks);
-            hipStreamSynchronize(s);
-            for (hipEvent_t e : evA) hipEventDestroy(e);
-            for (hipEvent_t e : evC) hipEventDestroy(e);
+            (void)hipStreamSynchronize(ks);
+            (void)hipStreamSynchronize(s);
+            for (hipEvent_t e : evA) (void)hipEventDestroy(e);
+            for (hipEvent_t e : evC) (void)hipEventDestroy(e);
         };
         rc = staged_h2d(dB, h.B, span(b_rows, h.ldb, b_len) * es, s);
         long prev0 = -1, prev1 = -1;
@@ -398,7 +398,7 @@ static int run_host(const Gemm<T> &h, int mode) {
             hipEvent_t ea = nullptr, ec = nullptr;
             if (hipEventCreateWithFlags(&ea, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&ec, hipEventDisableTiming) != hipSuccess) {
-                if (ea) hipEventDestroy(ea);
+                if (ea) (void)hipEventDestroy(ea);
                 rc = fail(UGEMM_ERR_HIP, "pipelined copy: events");
                 break;
             }
@@ -446,7 +446,7 @@ static int run_host(const Gemm<T> &h, int mode) {
     }
     if (!rc && need_c) rc = staged_h2d(dC, h.C, span(h.M, h.ldc, h.N) * es, s);
     if (rc) {
-        hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(s);
         return rc;
     }
     Gemm<T> d = h;
@@ -456,7 +456,7 @@ static int run_host(const Gemm<T> &h, int mode) {
     if constexpr (sizeof(T) == 4) rc = run_sgemm(d, mode, -1, s);
     else rc = run_dgemm(d, -1, s);
     if (rc) {
-        hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(s);
         return rc;
     }
     // Output: contiguous when C has no gaps; otherwise row by row, so that
@@ -710,14 +710,14 @@ int ugemm_hip_init(int device, size_t max_bytes) {
 void ugemm_hip_finish(void) {
     std::lock_guard<std::mutex> lk(g_dev.mu);
     if (!g_dev.ready) return;
-    hipSetDevice(g_dev.device);
-    if (g_dev.stream) hipStreamSynchronize(g_dev.stream);
+    (void)hipSetDevice(g_dev.device);
+    if (g_dev.stream) (void)hipStreamSynchronize(g_dev.stream);
     release_staging();
-    if (g_dev.staging) hipFree(g_dev.staging);
-    if (g_dev.stream) hipStreamDestroy(g_dev.stream);
+    if (g_dev.staging) (void)hipFree(g_dev.staging);
+    if (g_dev.stream) (void)hipStreamDestroy(g_dev.stream);
     if (g_dev.kstream) {
-        hipStreamSynchronize(g_dev.kstream);
-        hipStreamDestroy(g_dev.kstream);
+        (void)hipStreamSynchronize(g_dev.kstream);
+        (void)hipStreamDestroy(g_dev.kstream);
     }
     g_dev.kstream = nullptr;
     g_dev.staging = nullptr;

@@ -57,7 +57,7 @@ __global__ void fill_uniform(float *p, long n, unsigned seed) {
 
 void arch_info(char *arch, size_t n, int *cus) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
         snprintf(arch, n, "%s", prop.gcnArchName);
@@ -153,15 +153,15 @@ int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode
             for (int si = 0; si < nsplit; ++si) {
                 g.splits = split_set[si];
                 if (run_sgemm(g, mode, v, s)) continue;   // variant not applicable here
-                hipEventRecord(e0, s);
+                (void)hipEventRecord(e0, s);
                 for (int r = 0; r < reps; ++r) run_sgemm(g, mode, v, s);
-                hipEventRecord(e1, s);
+                (void)hipEventRecord(e1, s);
                 if ((e = hipEventSynchronize(e1))) {
                     rc = hip_fail(e, "tune: timing");
                     break;
                 }
                 float ms = 0;
-                hipEventElapsedTime(&ms, e0, e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
                 ms /= reps;
                 if (best < 0 || ms < best_ms) {
                     best = v;
@@ -171,15 +171,15 @@ int ugemm_hip_tune_sgemm(char transa, char transb, int M, int N, int K, int mode
             }
         }
     } while (0);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     if (s) {
-        hipStreamSynchronize(s);
-        hipStreamDestroy(s);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
     }
-    hipFree(A);
-    hipFree(B);
-    hipFree(C);
+    (void)hipFree(A);
+    (void)hipFree(B);
+    (void)hipFree(C);
     if (rc) return rc;
     if (best < 0) return fail(UGEMM_ERR_UNSUPPORTED, "tune: no variant ran");
     tune_record(mode, ta, tb, M, N, K, best, best_ms, best_splits);
