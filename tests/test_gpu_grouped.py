@@ -209,13 +209,13 @@ def uniform_b_cases(n, seed, tb, sizes=(7, 14, 28)):
 
 
 @pytest.mark.parametrize("mode", SPLIT_GROUPED)
-@pytest.mark.parametrize("variant", [7, 8])
+@pytest.mark.parametrize("variant", [8, 9])
 @pytest.mark.parametrize("tb", ["N", "T"])
 def test_grouped_split_forms_split_b(mode, variant, tb):
     """Grouped-only variants that split B in the core from its FP32 source
     (no B planes): every B of the plan has one layout; the form's bound; a
     captured plan replays bitwise identically."""
-    assert len(ug.grouped_variants(mode)) == 9 and ug.grouped_variants(mode)[variant].endswith("_split_b")
+    assert len(ug.grouped_variants(mode)) == 10 and ug.grouped_variants(mode)[variant].endswith("_split_b")
     cases = [DevCase(c) for c in uniform_b_cases(12, 70 + variant + 3 * mode + (tb == "T"), tb)]
     ug.force_variant(mode, variant)
     try:
@@ -241,7 +241,8 @@ def test_grouped_split_b_needs_one_b_layout():
     """A split-B variant on a plan whose B layouts differ is refused
     (UNSUPPORTED); the tuner skips such variants."""
     cases = [DevCase(c) for c in uneven_cases(6, 88)]
-    ug.force_variant(ug.MODE_BF16X3, 7)
+    split_b = len(ug.grouped_variants(ug.MODE_BF16X3)) - 2   # the first split-B variant
+    ug.force_variant(ug.MODE_BF16X3, split_b)
     try:
         with pytest.raises(ug.UgemmError) as e:
             ug.GroupedPlan([d.problem() for d in cases], mode=ug.MODE_BF16X3, capture=False)
@@ -249,7 +250,7 @@ def test_grouped_split_b_needs_one_b_layout():
     finally:
         ug.force_variant(ug.MODE_BF16X3, -1)
     plan = ug.GroupedPlan([d.problem() for d in cases], mode=ug.MODE_BF16X3, capture=False)
-    assert plan.tune() < 7
+    assert plan.tune() < split_b
     plan.destroy()
 
 

@@ -41,15 +41,15 @@ def test_split_layouts_shapes(mode, major, ta, tb):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("variant", range(7))
+@pytest.mark.parametrize("variant", range(8))
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_split_every_variant(mode, variant, ta, tb):
-    assert len(ug.variants(mode)) == 7
+    assert len(ug.variants(mode)) == 8
     ug.force_variant(mode, variant)
     try:
         # The last three have three or more 256-wide tiles on both sides (no
         # operand is split in the core) and 4, 1 and 2 K tiles of 32: the
-        # interleaved stage copy of variant 6 at its pipeline's edges.
+        # streamed staging of variants 6 and 7 at its pipeline's edges.
         for (M, N, K) in ((130, 250, 70), (300, 260, 96), (600, 700, 100), (520, 530, 20), (530, 600, 40)):
             case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=31 + variant)
             C = run_host(case, mode)
@@ -84,7 +84,7 @@ def test_split_variants_bitwise_identical(mode, shape, splits):
 
 
 @pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
-@pytest.mark.parametrize("variant", [4, 6])
+@pytest.mark.parametrize("variant", [4, 6, 7])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 @pytest.mark.parametrize("pad", [0, 1])
 def test_split_operand_fused_in_core(mode, variant, ta, tb, pad):

@@ -6,7 +6,7 @@ Checked here for every form that splits (FP32, the split forms, DGEMM):
 parity with the oracle to the same bounds as unsplit calls, bitwise equality
 of repeated calls (fixed reduction order), beta == 0 never reading C, the
 bias/LeakyReLU epilogue after the reduction, the automatic split of a small
-grid, and the tuner that times split counts (its version-2 cache file)."""
+grid, and the tuner that times split counts (its version-3 cache file)."""
 import os
 import zlib
 
@@ -49,7 +49,7 @@ def test_split_k_parity(mode, splits, ta, tb):
 
 
 @pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
-@pytest.mark.parametrize("variant", [4, 6])
+@pytest.mark.parametrize("variant", [4, 6, 7])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
 def test_split_k_with_eight_wave_variants(mode, variant, ta, tb):
     """Split K chunks that start at a non-zero K tile, through the 8-wave
@@ -152,7 +152,7 @@ def test_force_splits_argument_errors():
 
 def test_tuner_times_split_counts_and_persists_them(tmp_path):
     """ugemm_hip_tune_sgemm on a few-tile, long-K shape: the chosen variant is
-    recorded with a split count; the cache file is version 2 and loads back."""
+    recorded with a split count; the cache file is version 3 and loads back."""
     L = ug.load()
     M, N, K = 256, 192, 8192
     v = L.ugemm_hip_tune_sgemm(b"N", b"N", M, N, K, ug.MODE_FP32)
@@ -161,7 +161,7 @@ def test_tuner_times_split_counts_and_persists_them(tmp_path):
     path = os.path.join(str(tmp_path), "tune.txt")
     assert L.ugemm_hip_tune_save(path.encode()) == ug.OK
     lines = open(path).read().split("\n")
-    assert lines[0].startswith("ugemm-hip-tune 2 ")
+    assert lines[0].startswith("ugemm-hip-tune 3 ")
     rows = [l.split() for l in lines[1:] if l.strip()]
     mine = [r for r in rows if int(r[0]) == ug.MODE_FP32 and int(r[6]) == v]
     assert mine and all(len(r) == 9 and 0 <= int(r[8]) <= 64 for r in rows)

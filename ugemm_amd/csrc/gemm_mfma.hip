@@ -1179,7 +1179,9 @@ struct Table;
 // 15-20% faster core than V0).  V6: 8 waves with streamed reads, the
 // interleaved stage copy and (NP >= 2) the early barrier; 6-10% faster than
 // V5 at 8192^3.  NP = 1, 2: 256x256x32 tiles (32 / 64 KiB per stage); NP = 3:
-// 256x128x32 (72 KiB per stage).  Smaller tiles serve small grids.
+// 256x128x32 (72 KiB per stage).  V7: V6's schedule on 128x128 tiles (8
+// waves of 64x32) for grids too small for V6.  Smaller tiles serve small
+// grids.
 template <>
 struct Table<1> {
     using V0 = CoreCfg<256, 256, 32, 1>;
@@ -1189,6 +1191,7 @@ struct Table<1> {
     using V4 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 1, FmtBf16, 4, 4, true>;
     using V6 = CoreCfg<256, 256, 32, 1, FmtBf16, 2, 4, true, true, true>;
+    using V7 = CoreCfg<128, 128, 32, 1, FmtBf16, 2, 4, true, true, true>;
 };
 template <>
 struct Table<2> {
@@ -1199,6 +1202,7 @@ struct Table<2> {
     using V4 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 2, FmtBf16, 4, 4, true>;
     using V6 = CoreCfg<256, 256, 32, 2, FmtBf16, 2, 4, true, true, true, true>;
+    using V7 = CoreCfg<128, 128, 32, 2, FmtBf16, 2, 4, true, true, true, true>;
 };
 template <>
 struct Table<3> {
@@ -1209,6 +1213,7 @@ struct Table<3> {
     using V4 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true>;
     using V5 = CoreCfg<256, 128, 32, 3, FmtBf16, 4, 2, true>;
     using V6 = CoreCfg<256, 128, 32, 3, FmtBf16, 2, 4, true, true, true, true>;
+    using V7 = CoreCfg<128, 128, 32, 3, FmtBf16, 2, 4, true, true, true, true>;
 };
 template <>
 struct Table<2, FmtF16> {
@@ -1219,8 +1224,9 @@ struct Table<2, FmtF16> {
     using V4 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, false>;
     using V5 = CoreCfg<256, 256, 32, 2, FmtF16, 4, 4, true>;
     using V6 = CoreCfg<256, 256, 32, 2, FmtF16, 2, 4, true, true, true, true>;
+    using V7 = CoreCfg<128, 128, 32, 2, FmtF16, 2, 4, true, true, true, true>;
 };
-constexpr int kNumVariants = 7;
+constexpr int kNumVariants = 8;
 
 template <class CF>
 const char *cfg_name() {
@@ -1447,6 +1453,7 @@ int launch_np(const Gemm<float> &g, int variant, hipStream_t s) {
         case 4: return run_cfg<typename T::V4>(g, s);
         case 5: return run_cfg<typename T::V5>(g, s);
         case 6: return run_cfg<typename T::V6>(g, s);
+        case 7: return run_cfg<typename T::V7>(g, s);
         default: return fail(UGEMM_ERR_ARG, "split form: no variant %d", variant);
     }
 }
@@ -1511,6 +1518,7 @@ int launch_np_conv(const Gemm<float> &g, const ConvDesc &cd, int variant, hipStr
         case 4: return run_cfg_conv<typename T::V4>(g, cd, s);
         case 5: return run_cfg_conv<typename T::V5>(g, cd, s);
         case 6: return run_cfg_conv<typename T::V6>(g, cd, s);
+        case 7: return run_cfg_conv<typename T::V7>(g, cd, s);
         default: return fail(UGEMM_ERR_ARG, "split form: no variant %d", variant);
     }
 }
@@ -1526,6 +1534,7 @@ const char *name_np(int v) {
         case 4: return cfg_name<typename T::V4>();
         case 5: return cfg_name<typename T::V5>();
         case 6: return cfg_name<typename T::V6>();
+        case 7: return cfg_name<typename T::V7>();
         default: return nullptr;
     }
 }
@@ -1561,8 +1570,10 @@ int mfma_pick_variant(const Gemm<float> &g, int mode) {
         const bool fuse = np <= 2 && (tn <= 2 || tm <= 2);
         return fuse ? 4 : (np >= 2 ? 6 : 5);
     }
+    // 128 x 128 tiles: the 8-wave streamed V7 (2048^3: 12-24% faster per
+    // call than the 4-wave V1, whose one wave per SIMD hides no latency).
     const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
-    if (t128 >= cus / 2) return 1;
+    if (t128 >= cus / 2) return 7;
     return 3;
 }
 
@@ -1632,6 +1643,7 @@ int grouped_np(int variant, int b_kcont, const PrepJob *jobs, const int *block_j
         case 4: return grouped_core<typename T::V4>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         case 5: return grouped_core<typename T::V5>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         case 6: return grouped_core<typename T::V6>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
+        case 7: return grouped_core<typename T::V7>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc);
         default: return fail(UGEMM_ERR_ARG, "grouped split form: no variant %d", variant);
     }
 }
@@ -1646,7 +1658,8 @@ void tile_of(int variant, int *bm, int *bn) {
         case 3: *bm = T::V3::BM; *bn = T::V3::BN; break;
         case 4: *bm = T::V4::BM; *bn = T::V4::BN; break;
         case 5: *bm = T::V5::BM; *bn = T::V5::BN; break;
-        default: *bm = T::V6::BM; *bn = T::V6::BN; break;
+        case 6: *bm = T::V6::BM; *bn = T::V6::BN; break;
+        default: *bm = T::V7::BM; *bn = T::V7::BN; break;
     }
 }
 }  // namespace

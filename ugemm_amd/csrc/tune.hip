@@ -6,9 +6,11 @@
 // Bucket key: (mode, transa, transb, ceil(log2 M), ceil(log2 N), ceil(log2 K));
 // grouped plans use key TUNE_KEY_GROUPED with (count, MFLOP, 0) in place of
 // (M, N, K).
-// File format (version 2; version-1 files, which have no <splits> column,
-// are read with splits = 0, the built-in choice):
-//   ugemm-hip-tune 2 <gcnArch> <CUs>
+// File format (version 3; version-1 files, which have no <splits> column,
+// are read with splits = 0, the built-in choice; version 3 added a split-form
+// core variant, which renumbered the grouped split-B variants, so the
+// split-form entries of version-1 and -2 files are skipped):
+//   ugemm-hip-tune 3 <gcnArch> <CUs>
 //   <mode> <ta> <tb> <lgM> <lgN> <lgK> <variant> <ms> <splits>
 // Entries from a different architecture or CU count are ignored on load.
 #include "internal.h"
@@ -195,7 +197,7 @@ int ugemm_hip_tune_save(const char *path) {
     char arch[64];
     int cus;
     arch_info(arch, sizeof arch, &cus);
-    fprintf(f, "ugemm-hip-tune 2 %s %d\n", arch, cus);
+    fprintf(f, "ugemm-hip-tune 3 %s %d\n", arch, cus);
     {
         std::lock_guard<std::mutex> lk(g_mu);
         for (auto &kv : g_table) {
@@ -225,16 +227,21 @@ int tune_load_file(const char *path) {
     char arch[64], farch[64];
     int cus, fcus, ver;
     arch_info(arch, sizeof arch, &cus);
-    if (fscanf(f, "ugemm-hip-tune %d %63s %d", &ver, farch, &fcus) != 3 || (ver != 1 && ver != 2)) {
+    if (fscanf(f, "ugemm-hip-tune %d %63s %d", &ver, farch, &fcus) != 3 || ver < 1 || ver > 3) {
         fclose(f);
-        return fail(UGEMM_ERR_TUNE_FILE, "%s: not a version-1 or version-2 tuning cache", path);
+        return fail(UGEMM_ERR_TUNE_FILE, "%s: not a version-1, -2 or -3 tuning cache", path);
     }
+    auto split_form = [](int mode) {
+        return mode == UGEMM_MODE_BF16X1 || mode == UGEMM_MODE_BF16X3 || mode == UGEMM_MODE_BF16X6 ||
+               mode == UGEMM_MODE_FP16X3;
+    };
     int loaded = 0;
     if (!strcmp(arch, farch) && cus == fcus) {
         int m, a, b, lm, ln, lk, v, sp = 0;
         float ms;
         while (fscanf(f, "%d %d %d %d %d %d %d %f", &m, &a, &b, &lm, &ln, &lk, &v, &ms) == 8) {
-            if (ver == 2 && fscanf(f, "%d", &sp) != 1) break;
+            if (ver >= 2 && fscanf(f, "%d", &sp) != 1) break;
+            if (ver < 3 && split_form(m == TUNE_KEY_GROUPED ? b : m)) continue;   // older variant numbering
             const int nv = (m == TUNE_KEY_GROUPED) ? grouped_variant_count(b)
                                                    : ((m < -1 || m >= UGEMM_MODE_COUNT) ? 0 : ugemm_hip_variant_count(m));
             if (v < 0 || v >= nv || sp < 0 || sp > 64) continue;
