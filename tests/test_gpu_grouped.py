@@ -209,13 +209,13 @@ def uniform_b_cases(n, seed, tb, sizes=(7, 14, 28)):
 
 
 @pytest.mark.parametrize("mode", SPLIT_GROUPED)
-@pytest.mark.parametrize("variant", [8, 9])
+@pytest.mark.parametrize("variant", [8, 9, 10])
 @pytest.mark.parametrize("tb", ["N", "T"])
 def test_grouped_split_forms_split_b(mode, variant, tb):
     """Grouped-only variants that split B in the core from its FP32 source
     (no B planes): every B of the plan has one layout; the form's bound; a
     captured plan replays bitwise identically."""
-    assert len(ug.grouped_variants(mode)) == 10 and ug.grouped_variants(mode)[variant].endswith("_split_b")
+    assert len(ug.grouped_variants(mode)) == 11 and ug.grouped_variants(mode)[variant].endswith("_split_b")
     cases = [DevCase(c) for c in uniform_b_cases(12, 70 + variant + 3 * mode + (tb == "T"), tb)]
     ug.force_variant(mode, variant)
     try:
@@ -241,7 +241,7 @@ def test_grouped_split_b_needs_one_b_layout():
     """A split-B variant on a plan whose B layouts differ is refused
     (UNSUPPORTED); the tuner skips such variants."""
     cases = [DevCase(c) for c in uneven_cases(6, 88)]
-    split_b = len(ug.grouped_variants(ug.MODE_BF16X3)) - 2   # the first split-B variant
+    split_b = len(ug.variants(ug.MODE_BF16X3))   # the first split-B variant
     ug.force_variant(ug.MODE_BF16X3, split_b)
     try:
         with pytest.raises(ug.UgemmError) as e:
@@ -342,8 +342,8 @@ def test_grouped_mode_variant_tables():
     for mode in (ug.MODE_BF16X1, ug.MODE_BF16X3, ug.MODE_BF16X6, ug.MODE_FP16X3):
         # the single-GEMM split-core table, then the grouped-only split-B variants
         g, v = ug.grouped_variants(mode), ug.variants(mode)
-        assert g[:len(v)] == v and len(g) == len(v) + 2
-        assert g[len(v)] == v[1] + "_split_b"
+        assert g[:len(v)] == v and len(g) == len(v) + 3
+        assert g[len(v)] == v[1] + "_split_b" and g[len(v) + 2] == v[7] + "_split_b"
     for mode in (ug.MODE_FP8_BLOCK, 42, -1):
         assert L.ugemm_hip_grouped_mode_variant_count(mode) == ug.ERR_ARG
     assert L.ugemm_hip_grouped_mode_variant_name(ug.MODE_FP32_MFMA, 5) is None

@@ -83,19 +83,23 @@ def test_split_variants_bitwise_identical(mode, shape, splits):
         assert np.array_equal(outs[0].view(np.uint32), outs[v].view(np.uint32)), f"variant {v} differs from 0"
 
 
-@pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1])
+@pytest.mark.parametrize("mode", [ug.MODE_BF16X3, ug.MODE_FP16X3, ug.MODE_BF16X1, ug.MODE_BF16X6])
 @pytest.mark.parametrize("variant", [4, 6, 7])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 @pytest.mark.parametrize("pad", [0, 1])
 def test_split_operand_fused_in_core(mode, variant, ta, tb, pad):
     """The 8-wave variants split an operand that at most two tiles re-read
     while staging it, from either FP32 layout: k-contiguous (A in N, B in T)
-    or row-contiguous (A in T, B in N), the latter transposed in registers.
-    One side of each shape has at most 2 tiles of 256, the other 5; odd row
-    counts and pad = 1 (odd leading dimensions: scalar loads) hit the edges."""
+    or row-contiguous (A in T, B in N), the latter transposed in registers
+    (two rows per thread with 256-row tiles, one with 128-row tiles: the
+    128 x 128 variant 7 and the 256 x 128 tiles of BF16x6).  One side of
+    each shape has at most 2 tiles of 256 (the last two: of 128), the
+    other 5 or more; odd row counts and pad = 1 (odd leading dimensions:
+    scalar loads) hit the edges."""
     ug.force_variant(mode, variant)
     try:
-        for (M, N, K) in ((301, 1100, 77), (1100, 257, 96), (3, 1031, 40), (1030, 511, 33)):
+        for (M, N, K) in ((301, 1100, 77), (1100, 257, 96), (3, 1031, 40), (1030, 511, 33), (1100, 200, 77),
+                          (250, 1031, 96)):
             case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=M * 3 + N + K + pad, pad=pad)
             C = run_host(case, mode)
             ok, worst = case.judge(C, mode=mode)

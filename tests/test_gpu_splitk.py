@@ -55,11 +55,11 @@ def test_split_k_with_eight_wave_variants(mode, variant, ta, tb):
     """Split K chunks that start at a non-zero K tile, through the 8-wave
     variants: the interleaved stage copy (variant 6, 3 x 3 tiles) and an
     operand split in the core from either FP32 layout (300 or 1100 rows: one
-    side of at most two 256-wide tiles)."""
+    side of at most two 256-wide tiles; 250: of two 128-wide tiles)."""
     ug.force_variant(mode, variant)
     ug.force_splits(mode, 3)
     try:
-        for (M, N, K) in ((600, 700, 3000), (300, 1100, 2500), (1100, 257, 2100)):
+        for (M, N, K) in ((600, 700, 3000), (300, 1100, 2500), (1100, 257, 2100), (250, 1100, 2500)):
             case = Case("R", ta, tb, M, N, K, 1.5, 0.5, seed=M + N + K + variant)
             C = run_host(case, mode)
             ok, worst = case.judge(C, mode=mode)

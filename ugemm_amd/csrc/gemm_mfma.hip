@@ -39,6 +39,7 @@
 // l%16.  A and B use the same k assignment, so any order of the hardware's
 // internal sum pairs the right elements.
 #include <string>
+#include <type_traits>
 #include <utility>
 
 #include "internal.h"
@@ -733,35 +734,44 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
     constexpr int B_CH = NP * BN * CPR / CF::NT;          // 16-byte chunks per thread from B planes
     constexpr int FA_CH = BM * CPR / CF::NT, FB_CH = BN * CPR / CF::NT;   // fused chunk rows per thread
     static_assert(CPR == 4, "fused staging assumes 32-wide K tiles");
-    static_assert((FA != FUSE_RCONT || FA_CH == 2) && (FB != FUSE_RCONT || FB_CH == 2),
-                  "row-contiguous fusion: two rows per thread");
+    static_assert((FA != FUSE_RCONT || FA_CH == 1 || FA_CH == 2) && (FB != FUSE_RCONT || FB_CH == 1 || FB_CH == 2),
+                  "row-contiguous fusion: one or two rows per thread");
     V8 pa[FA ? 1 : A_CH], pb[FB ? 1 : B_CH];
     float fa[FA ? FA_CH : 1][8], fb[FB ? FB_CH : 1][8];
-    // Row-contiguous source: rows [r, r+2) of k-lines k .. k+7 into v[i][j]
-    // (row r+i, k+j); zeros outside rows x K.  vec: 8-byte loads allowed.
-    // full: the whole stage lies inside rows x K and vector loads are
-    // allowed (uniform over the block): unguarded loads, no per-element
-    // branches.
-    auto load_rc = [&](const float *X, long ld, int rows, int r, int k, bool vec, float (&v)[2][8], bool full) {
+    // Row-contiguous source: rows [r, r+RW) of k-lines k .. k+7 into v[i][j]
+    // (row r+i, k+j); zeros outside rows x K.  RW is 1 or 2 (stage rows per
+    // thread); vec: 8-byte loads allowed (RW == 2).  full: the whole stage
+    // lies inside rows x K and vector loads are allowed (uniform over the
+    // block): unguarded loads, no per-element branches.
+    auto load_rc = [&](const float *X, long ld, int rows, int r, int k, bool vec, auto &v, bool full) {
+        constexpr int RW = std::extent<std::remove_reference_t<decltype(v)>>::value;
         if (full) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const float2 x = *reinterpret_cast<const float2 *>(X + (long)(k + j) * ld + r);
-                v[0][j] = x.x;
-                v[1][j] = x.y;
+                if constexpr (RW == 2) {
+                    const float2 x = *reinterpret_cast<const float2 *>(X + (long)(k + j) * ld + r);
+                    v[0][j] = x.x;
+                    v[1][j] = x.y;
+                } else {
+                    v[0][j] = X[(long)(k + j) * ld + r];
+                }
             }
             return;
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const float *p = X + (long)(k + j) * ld + r;
-            if (k + j < K && r + 2 <= rows && vec) {
-                const float2 x = *reinterpret_cast<const float2 *>(p);
-                v[0][j] = x.x;
-                v[1][j] = x.y;
+            if constexpr (RW == 2) {
+                if (k + j < K && r + 2 <= rows && vec) {
+                    const float2 x = *reinterpret_cast<const float2 *>(p);
+                    v[0][j] = x.x;
+                    v[1][j] = x.y;
+                } else {
+                    v[0][j] = (k + j < K && r < rows) ? p[0] : 0.0f;
+                    v[1][j] = (k + j < K && r + 1 < rows) ? p[1] : 0.0f;
+                }
             } else {
                 v[0][j] = (k + j < K && r < rows) ? p[0] : 0.0f;
-                v[1][j] = (k + j < K && r + 1 < rows) ? p[1] : 0.0f;
             }
         }
     };
@@ -799,8 +809,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
         const int k0 = kt * BK;
         const bool kin = k0 + BK <= K;
         if constexpr (FA == FUSE_RCONT) {
-            load_rc(Af, lda, M, m0 + (t % (BM / 2)) * 2, k0 + (t / (BM / 2)) * 8, vec_ab & 1, fa,
-                    kin && (vec_ab & 1) && m0 + BM <= M);
+            load_rc(Af, lda, M, m0 + (t % (BM / FA_CH)) * FA_CH, k0 + (t / (BM / FA_CH)) * 8, vec_ab & 1, fa,
+                    kin && (FA_CH == 1 || (vec_ab & 1)) && m0 + BM <= M);
         } else if constexpr (FA == FUSE_KCONT) {
             const bool full = kin && (vec_ab & 1) && m0 + BM <= M;
 #pragma unroll
@@ -813,8 +823,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
             for (int i = 0; i < A_CH; ++i) load_chunk(kt, i);
         }
         if constexpr (FB == FUSE_RCONT) {
-            load_rc(Bf, ldb, N, n0 + (t % (BN / 2)) * 2, k0 + (t / (BN / 2)) * 8, (vec_ab & 2) != 0, fb,
-                    kin && (vec_ab & 2) && n0 + BN <= N);
+            load_rc(Bf, ldb, N, n0 + (t % (BN / FB_CH)) * FB_CH, k0 + (t / (BN / FB_CH)) * 8, (vec_ab & 2) != 0, fb,
+                    kin && (FB_CH == 1 || (vec_ab & 2)) && n0 + BN <= N);
         } else if constexpr (FB == FUSE_KCONT) {
             const bool full = kin && (vec_ab & 2) && n0 + BN <= N;
 #pragma unroll
@@ -849,8 +859,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
         E *a_reg = lds + buf * STAGE, *b_reg = a_reg + NP * CF::A_PLANE;
         if constexpr (FA == FUSE_RCONT) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = (t % (BM / 2)) * 2 + i, f = r * 4 + ((t / (BM / 2)) ^ ((r >> 1) & 3));
+            for (int i = 0; i < FA_CH; ++i) {
+                const int r = (t % (BM / FA_CH)) * FA_CH + i, f = r * 4 + ((t / (BM / FA_CH)) ^ ((r >> 1) & 3));
                 store_fused(a_reg, CF::A_PLANE, amax_a, m0 + r, f, fa[i]);
             }
         } else if constexpr (FA == FUSE_KCONT) {
@@ -865,8 +875,8 @@ __global__ __launch_bounds__(CF::NT) void split_nt_core(int M_, int N_, int Kp_,
         }
         if constexpr (FB == FUSE_RCONT) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = (t % (BN / 2)) * 2 + i, f = r * 4 + ((t / (BN / 2)) ^ ((r >> 1) & 3));
+            for (int i = 0; i < FB_CH; ++i) {
+                const int r = (t % (BN / FB_CH)) * FB_CH + i, f = r * 4 + ((t / (BN / FB_CH)) ^ ((r >> 1) & 3));
                 store_fused(b_reg, CF::B_PLANE, amax_b, n0 + r, f, fb[i]);
             }
         } else if constexpr (FB == FUSE_KCONT) {
@@ -1368,9 +1378,10 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
     // layout the core streams contiguously (measured: splitting a re-read
     // operand in the core is ~50% slower at 8192^3).
     // A row-contiguous source (A transposed, B not) is transposed in
-    // registers on the way; that needs two rows per thread.  At most one
+    // registers on the way; that needs one or two rows per thread.  At most one
     // operand is fused per call (the larger, when both qualify).
-    constexpr bool RC_A = CF::BM * CF::CPR / CF::NT == 2, RC_B = CF::BN * CF::CPR / CF::NT == 2;
+    constexpr int RW_A = CF::BM * CF::CPR / CF::NT, RW_B = CF::BN * CF::CPR / CF::NT;
+    constexpr bool RC_A = RW_A == 1 || RW_A == 2, RC_B = RW_B == 1 || RW_B == 2;
     bool fa = CF::FUSABLE && Np / CF::BN <= 2 && (!g.ta || RC_A);
     bool fb = CF::FUSABLE && Mp / CF::BM <= 2 && (g.tb || RC_B);
     if (fa && fb) (g.M >= g.N ? fb : fa) = false;
@@ -1614,9 +1625,10 @@ int grouped_core(const PrepJob *jobs, const int *block_job, int prep_blocks, con
 
 // Grouped-only variants (kNumVariants + i): base variant split_b_base(np, i)
 // with B split in the core -- the 4-wave tiles with 128 B rows (each thread
-// stages two rows of a row-contiguous B): 128 x 128, and 256 x 128.
-constexpr int kNumSplitB = 2;
-constexpr int split_b_base(int np, int i) { return i == 0 ? 1 : (np == 3 ? 0 : 2); }
+// stages two rows of a row-contiguous B): 128 x 128, and 256 x 128; then
+// the 8-wave streamed 128 x 128 (one row per thread).
+constexpr int kNumSplitB = 3;
+constexpr int split_b_base(int np, int i) { return i == 0 ? 1 : i == 2 ? 7 : (np == 3 ? 0 : 2); }
 
 template <int NP, class FMT = FmtBf16>
 int grouped_np(int variant, int b_kcont, const PrepJob *jobs, const int *block_job, int prep_blocks,
@@ -1626,12 +1638,14 @@ int grouped_np(int variant, int b_kcont, const PrepJob *jobs, const int *block_j
 #define UGEMM_SPLIT_B(CFG)                                                                                    \
     (b_kcont ? grouped_core<CFG, FUSE_KCONT>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc)           \
              : grouped_core<CFG, FUSE_RCONT>(jobs, block_job, prep_blocks, probs, tiles, ntiles, s, sc))
-    static_assert(split_b_base(NP, 0) == 1 && split_b_base(NP, 1) == (NP == 3 ? 0 : 2), "split-B table");
+    static_assert(split_b_base(NP, 0) == 1 && split_b_base(NP, 1) == (NP == 3 ? 0 : 2) && split_b_base(NP, 2) == 7,
+                  "split-B table");
     switch (variant) {
         case kNumVariants + 0: return UGEMM_SPLIT_B(typename T::V1);
         case kNumVariants + 1:
             if constexpr (NP == 3) return UGEMM_SPLIT_B(typename T::V0);
             else return UGEMM_SPLIT_B(typename T::V2);
+        case kNumVariants + 2: return UGEMM_SPLIT_B(typename T::V7);
         default: break;
     }
 #undef UGEMM_SPLIT_B
@@ -1677,9 +1691,10 @@ const char *mfma_grouped_variant_name(int mode, int variant) {
     static const std::string names[UGEMM_MODE_COUNT][kNumSplitB] = {
 #define UGEMM_N(m, i) (std::string(mfma_variant_name(m, split_b_base(planes_of(m), i)) \
                                     ? mfma_variant_name(m, split_b_base(planes_of(m), i)) : "") + "_split_b")
-        {UGEMM_N(0, 0), UGEMM_N(0, 1)}, {UGEMM_N(1, 0), UGEMM_N(1, 1)}, {UGEMM_N(2, 0), UGEMM_N(2, 1)},
-        {UGEMM_N(3, 0), UGEMM_N(3, 1)}, {UGEMM_N(4, 0), UGEMM_N(4, 1)}, {UGEMM_N(5, 0), UGEMM_N(5, 1)},
-        {UGEMM_N(6, 0), UGEMM_N(6, 1)}
+        {UGEMM_N(0, 0), UGEMM_N(0, 1), UGEMM_N(0, 2)}, {UGEMM_N(1, 0), UGEMM_N(1, 1), UGEMM_N(1, 2)},
+        {UGEMM_N(2, 0), UGEMM_N(2, 1), UGEMM_N(2, 2)}, {UGEMM_N(3, 0), UGEMM_N(3, 1), UGEMM_N(3, 2)},
+        {UGEMM_N(4, 0), UGEMM_N(4, 1), UGEMM_N(4, 2)}, {UGEMM_N(5, 0), UGEMM_N(5, 1), UGEMM_N(5, 2)},
+        {UGEMM_N(6, 0), UGEMM_N(6, 1), UGEMM_N(6, 2)}
 #undef UGEMM_N
     };
     return names[mode][variant - kNumVariants].c_str();

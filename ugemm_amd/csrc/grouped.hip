@@ -590,9 +590,11 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
 int default_split_variant(const ugemm_grouped_plan *p) {
     // Most of the work in problems of 128+ rows and columns: 128 x 128 tiles; otherwise 64 x 64.
     // With 128 x 128 tiles, every B in one layout and few tile rows per
-    // problem (B split again by each, on average at most 4), the variant that
-    // splits B in the core: B's planes would cost more than re-splitting
-    // (config 4: 4.7 against 5.8 ms for BF16x3).
+    // problem (B split again by each, on average at most 4), the 8-wave
+    // streamed variant that splits B in the core: B's planes would cost more
+    // than re-splitting (512 im2col-shaped problems, M = 32..512,
+    // N = 49..3136, K = 144..4608: 4.1 ms, against 4.7 for the 4-wave
+    // split-B variant and 5.8 with planes, for BF16x3).
     double big = 0, all = 0, rows_w = 0;
     int n_t = 0, n_n = 0;
     for (const DevProblem &q : p->probs) {
@@ -603,7 +605,7 @@ int default_split_variant(const ugemm_grouped_plan *p) {
         if (q.M > 0 && q.N > 0) ((q.layout & 2) ? n_t : n_n)++;
     }
     if (all == 0 || big < 0.5 * all) return 3;
-    const int split_b = mfma_grouped_variant_count(p->mode) - 2;   // the 128 x 128 split-B variant
+    const int split_b = mfma_variant_count(p->mode) + 2;   // split-B on the 8-wave streamed 128 x 128
     if (!(n_t && n_n) && rows_w <= 4.0 * all && mfma_grouped_splits_b(p->mode, split_b)) return split_b;
     return 1;
 }
