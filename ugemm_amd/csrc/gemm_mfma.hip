@@ -669,8 +669,9 @@ enum { FUSE_NONE = 0, FUSE_KCONT = 1, FUSE_RCONT = 2 };
 //     (row r = f / 4, stored chunk f % 4, which holds k-chunk (f % 4) ^ sw(r)).
 //   FUSE_RCONT: split while staging from the row-contiguous FP32 source
 //     X[k*ld + r], transposed in registers: thread t loads 8 k-lines of the
-//     RPT = 2 adjacent rows rp*2, rp*2 + 1 (rp = t % (rows/2), k-chunk
-//     t / (rows/2)), coalesced along r, and splits one 8-k chunk per row.
+//     RPT adjacent rows rp*RPT .. rp*RPT + RPT - 1 (RPT = rows*4/NT, 1 or 2;
+//     rp = t % (rows/RPT), k-chunk t / (rows/RPT)), coalesced along r, and
+//     splits one 8-k chunk per row.
 // Either way each split chunk lands where the plane copy would have put it
 // (zeros beyond M, N, K).
 // GR (grouped plans): block b computes tile gtiles[b] of problem
