@@ -210,8 +210,11 @@ const char *ugemm_hip_grouped_variant_name(int variant);
 /* The same per plan mode: UGEMM_MODE_FP32 (the variants above),
  * UGEMM_MODE_FP32_MFMA (the same five tile choices on the FP32 matrix
  * cores) and the split forms (their split-core variants, as
- * ugemm_hip_variant_name).  Count: UGEMM_ERR_ARG for a mode grouped plans
- * do not provide; name: NULL out of range. */
+ * ugemm_hip_variant_name, then three grouped-only variants, names ending
+ * in "_split_b", that split B in the core from its FP32 source: plans whose
+ * every B has one layout; UGEMM_ERR_UNSUPPORTED otherwise).  Count:
+ * UGEMM_ERR_ARG for a mode grouped plans do not provide; name: NULL out of
+ * range. */
 int ugemm_hip_grouped_mode_variant_count(int mode);
 const char *ugemm_hip_grouped_mode_variant_name(int mode, int variant);
 
