@@ -144,6 +144,31 @@ def timed(fn, steps, warmup, stream, warm_ms=0.0):
     return ms, (core_ms / n if n else None), wall
 
 
+def mfma_sustained(rounds=20000):
+    """The device's sustained 16x16x32 MFMA rate (TFLOP/s) for bf16 and f16
+    operands: every SIMD issues nothing but independent MFMAs for ~25 ms
+    (ugemm_hip_mfma_ceiling), the rate any split-form core can reach under the
+    power limit.  The operands are planes of the headline's data: U[-1,1)
+    split into hi + lo (bf16, round to nearest even; fp16 after the 2^15 row
+    scale FP16X3 applies), since the rate depends on the bits multiplied."""
+    g = np.random.Generator(np.random.PCG64(0x55474D4D0C))
+    x = uniform(g, 4096).astype(np.float32)
+    u = x.view(np.uint32)
+    hi = ((u + np.uint32(0x7FFF) + ((u >> 16) & 1)) & np.uint32(0xFFFF0000)).view(np.float32)
+    r = (x - hi).view(np.uint32)
+    lo = ((r + np.uint32(0x7FFF) + ((r >> 16) & 1)) & np.uint32(0xFFFF0000))
+    bf = np.concatenate([(hi.view(np.uint32) >> 16), lo >> 16]).astype(np.uint16)
+    xs = x * np.float32(32768.0)
+    h16 = xs.astype(np.float16)
+    f16 = np.concatenate([h16, (xs - h16.astype(np.float32)).astype(np.float16)]).view(np.uint16)
+    out = {}
+    for name, planes, fp16 in (("bf16", bf, False), ("f16", f16, True)):
+        d = hip.DeviceArray.from_host(planes)
+        out[name] = round(max(ug.mfma_ceiling(d.ptr, fp16, rounds) for _ in range(2)), 1)
+        d.free()
+    return out
+
+
 def warm(fn, stream, ms=250.0):
     """Untimed calls of fn until ms of wall time has passed (see timed)."""
     t0 = time.perf_counter()
@@ -935,6 +960,13 @@ def main():
             "achieved_is": "algorithmic 2*M*N*K flop per launch / average core-kernel duration (HIP events)",
             "mfma_issue_frac": round(core_tf * mult / PEAK[peak_key], 4),
             "frac_of_fp32_vector_peak": round(core_tf / PEAK["fp32_vector"], 4)}
+    sustained = mfma_sustained()
+    elem = {"fp16x3": "f16", "bf16x3": "bf16", "bf16x6": "bf16", "bf16x1": "bf16"}.get(head)
+    if elem:
+        # What the core reaches of the matrix-core rate this device sustains
+        # under its power limit (all three products of the split counted).
+        roof["sustained_mfma_peak"] = sustained[elem]
+        roof["mfma_issue_frac_of_sustained"] = round(core_tf * mult / sustained[elem], 4)
     cnt = load_counters(head)
     if cnt:
         # From profiles/gemm_flop_counters.md: the clock the kernel runs at under
@@ -959,6 +991,8 @@ def main():
                                 "in forms with its error ratios"},
         "forms": res,
         "roofline": roof,
+        "mfma_sustained_tflops": dict(sustained, note="16x16x32 MFMAs only, on every SIMD, operands = split "
+                                      "planes of U[-1,1) data; published dense peak 2516.6"),
     }
     if target:
         line["speedup_vs_plain_fp32"] = {k: round(v["tflops"] / res["fp32"]["tflops"], 3)

@@ -306,6 +306,17 @@ const char *ugemm_hip_version(void);
 int ugemm_hip_timing_enable(int on);
 int ugemm_hip_timing_collect(double *total_ms);
 
+/* Sustained matrix-core rate of the device (diagnostic, no counterpart in
+ * the reference): every SIMD issues nothing but independent
+ * v_mfma_f32_16x16x32 instructions (f16 when fp16 != 0, else bf16) for
+ * `rounds` rounds of 16, on operands taken from d_operands, 1024 16-byte
+ * fragments (8192 16-bit values) in device memory -- planes of real data,
+ * since the rate under the power limit depends on the bits multiplied.
+ * *tflops receives the rate; it synchronizes the device.  It bounds what
+ * a GEMM core built on these MFMAs can reach on this device; the published
+ * peak assumes the top clock. */
+int ugemm_hip_mfma_ceiling(const void *d_operands, int fp16, int rounds, double *tflops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,7 @@ def load(path=None):
         "ugemm_hip_dgemv_dev": (i, [c, i, i, d, vp, i, vp, i, d, vp, i, vp]),
         "ugemm_hip_timing_enable": (i, [i]),
         "ugemm_hip_timing_collect": (i, [ctypes.POINTER(ctypes.c_double)]),
+        "ugemm_hip_mfma_ceiling": (i, [ctypes.c_void_p, i, i, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -275,6 +276,15 @@ def gemv_dev(trans, M, N, alpha, dA, lda, dx, incx, beta, dy, incy, double=False
 def timing_enable(on=True):
     """Bracket the core kernel of every following call (this thread) with HIP events."""
     load().ugemm_hip_timing_enable(1 if on else 0)
+
+
+def mfma_ceiling(d_operands, fp16, rounds=20000):
+    """Sustained 16x16x32 MFMA rate in TFLOP/s (f16 if fp16 else bf16) on the
+    1024 16-byte operand fragments at device address d_operands."""
+    tf = ctypes.c_double(0.0)
+    _raise_if(load().ugemm_hip_mfma_ceiling(ctypes.c_void_p(d_operands), 1 if fp16 else 0, int(rounds),
+                                           ctypes.byref(tf)))
+    return tf.value
 
 
 def timing_collect():

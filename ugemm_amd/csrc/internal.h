@@ -173,6 +173,11 @@ int launch_scale_f64(int M, int N, double beta, double *C, long ldc, hipStream_t
 // ---- BLAS-1/2 helpers (blas12.hip): device pointers, stream-ordered ----
 int launch_saxpy(int N, float alpha, const float *x, int incx, float *y, int incy, hipStream_t s);
 int launch_daxpy(int N, double alpha, const double *x, int incx, double *y, int incy, hipStream_t s);
+// ---- device diagnostics (diag.hip) ----
+// Sustained 16x16x32 MFMA rate (fp16 != 0: f16, else bf16) on operands taken
+// from 1024 16-byte fragments at ops (device memory), in TFLOP/s.
+int mfma_ceiling(const void *ops, int fp16, int rounds, double *tflops);
+
 int launch_saxpy3(int N, float a, const float *x, const float *y, float *z, hipStream_t s);
 int launch_daxpy3(int N, double a, const double *x, const double *y, double *z, hipStream_t s);
 int launch_sgemv(char trans, int M, int N, float alpha, const float *A, int lda, const float *x, int incx, float beta,

@@ -926,6 +926,12 @@ int ugemm_hip_dgemm_dev(char major, char transa, char transb, int M, int N, int 
     return rc ? rc : ok();
 }
 
+int ugemm_hip_mfma_ceiling(const void *d_operands, int fp16, int rounds, double *tflops) {
+    int rc = ensure_device();
+    if (!rc) rc = mfma_ceiling(d_operands, fp16, rounds, tflops);
+    return rc ? rc : ok();
+}
+
 int ugemm_hip_timing_enable(int on) {
     TimingState &T = t_timing;
     T.on = on != 0;
