@@ -232,54 +232,64 @@ __global__ __launch_bounds__(256) void split_prep_kcont_tiles(const float *__res
 }
 
 // FP16X3, k-contiguous source with k_pad <= 8192: the row maximum and the
-// split in one pass.  Block r reads row r once into registers (at most 4
-// chunks of 8 per thread), reduces its largest magnitude (amax[r], as the
-// separate row-maximum pass computes it: NaN ignored, 0 for padding rows),
-// then scales by 2^-row_exp and writes the planes -- the operand is read
-// once instead of twice.
+// split in one pass.  Block b reads rows r0 = 2b and r0 + 1 once into
+// registers, reduces each row's largest magnitude (amax[r], as the separate
+// row-maximum pass computes it: NaN ignored, 0 for padding rows), then
+// scales by 2^-row_exp and writes the planes -- the operand is read once
+// instead of twice.  Two rows per block, so that their 64-byte segments of
+// a K tile -- adjacent in the blocked layout -- leave as one 128-byte line
+// (one row per block: 0.172 against 0.138 ms at 8192^3 for the launch below
+// that also scans B): thread t takes 8-k chunk t % 4 of row r0 + (t / 4) % 2
+// in K tile t / 8 of each 32-tile pass (at most 8 passes, k_pad <= 8192).
+// Each row's maximum is reduced over the lanes that hold it (shuffles that
+// keep bit 2 of the lane fixed), then over the waves.
 template <int NP, class FMT>
-__device__ __forceinline__ void prep_kcont_rowscaled_row(const float *__restrict__ X, long ld, int rows, int K,
-                                                         bool vec, typename FMT::E *__restrict__ out,
-                                                         long plane_stride, int rows_pad, int k_pad,
-                                                         unsigned *__restrict__ amax, int r) {
+__device__ __forceinline__ void prep_kcont_rowscaled_pair(const float *__restrict__ X, long ld, int rows, int K,
+                                                          bool vec, typename FMT::E *__restrict__ out,
+                                                          long plane_stride, int rows_pad, int k_pad,
+                                                          unsigned *__restrict__ amax, int r0) {
     static_assert(FMT::SCALED, "row scales");
-    constexpr int MAXC = 4;   // chunks per thread: k_pad <= 4 * 256 * 8
+    constexpr int PASSES = 8;   // 8 passes x 32 K tiles x 32 = 8192
     const int t = threadIdx.x;
-    const int nchunk = k_pad / 8;
-    float x[MAXC][8];
+    const int ktl = t >> 3, rr = (t >> 2) & 1, cc = t & 3, r = r0 + rr;
+    const int ktiles = k_pad / 32;
+    float x[PASSES][8];
     float m = 0.0f;
 #pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
-        const int k = (t + i * 256) * 8;
+    for (int i = 0; i < PASSES; ++i) {
+        const int kt = i * 32 + ktl, k = kt * 32 + cc * 8;
         const float *p = X + (long)r * ld + k;
-        if (t + i * 256 < nchunk && r < rows && k + 8 <= K && vec) {
+        if (kt < ktiles && r < rows && k + 8 <= K && vec) {
             const float4 a = *reinterpret_cast<const float4 *>(p);
             const float4 c = *reinterpret_cast<const float4 *>(p + 4);
             x[i][0] = a.x; x[i][1] = a.y; x[i][2] = a.z; x[i][3] = a.w;
             x[i][4] = c.x; x[i][5] = c.y; x[i][6] = c.z; x[i][7] = c.w;
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[i][j] = (t + i * 256 < nchunk && r < rows && k + j < K) ? p[j] : 0.0f;
+            for (int j = 0; j < 8; ++j) x[i][j] = (kt < ktiles && r < rows && k + j < K) ? p[j] : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) m = fmaxf(m, fabsf(x[i][j]));
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    __shared__ float wmax[4];
-    if ((t & 63) == 0) wmax[t >> 6] = m;
+    m = fmaxf(m, __shfl_xor(m, 1));
+    m = fmaxf(m, __shfl_xor(m, 2));
+    m = fmaxf(m, __shfl_xor(m, 8));
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    __shared__ float wmax[4][2];
+    if ((t & 63) < 8 && (t & 3) == 0) wmax[t >> 6][rr] = m;
     __syncthreads();
-    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    m = fmaxf(fmaxf(wmax[0][rr], wmax[1][rr]), fmaxf(wmax[2][rr], wmax[3][rr]));
     const unsigned bits = m > 0.0f ? __float_as_uint(m) : 0u;
-    if (t == 0) amax[r] = bits;
+    if (t < 8 && (t & 3) == 0) amax[r] = bits;
     const int e = -row_exp(bits);
 #pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
-        const int c = t + i * 256;
-        if (c >= nchunk) break;
+    for (int i = 0; i < PASSES; ++i) {
+        const int kt = i * 32 + ktl;
+        if (kt >= ktiles) break;
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[i][j] = ldexpf(x[i][j], e);
-        write_planes<NP, FMT>(out + blocked_offset(r, c * 8, rows_pad), plane_stride, x[i]);
+        write_planes<NP, FMT>(out + blocked_offset(r, kt * 32 + cc * 8, rows_pad), plane_stride, x[i]);
     }
 }
 
@@ -288,7 +298,8 @@ __global__ __launch_bounds__(256) void split_prep_kcont_rowscaled(const float *_
                                                                   int K, bool vec, typename FMT::E *__restrict__ out,
                                                                   long plane_stride, int rows_pad, int k_pad,
                                                                   unsigned *__restrict__ amax) {
-    prep_kcont_rowscaled_row<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax, blockIdx.x);
+    prep_kcont_rowscaled_pair<NP, FMT>(X, ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax,
+                                       (int)blockIdx.x * 2);
 }
 
 // Row-contiguous source (X[k*ld + r]): the 64 x 64 tile (rows r0.., k k0..)
@@ -510,10 +521,11 @@ __global__ __launch_bounds__(256) void row_amax_rcont(const float *__restrict__ 
 // out = its maxima, nbx = K chunks per row (k-contiguous: 2048 wide) or per
 // 256-column block (row-contiguous: kAmaxRcontSpan deep)), blocks as in the kernels above.
 // FP16X3 with k-contiguous op(A) (K <= 8192) and row-contiguous op(B): A's
-// fused row maximum + split (one block per padded row of A) and B's column
+// fused row maximum + split (one block per two padded rows of A) and B's column
 // maxima (row_amax_rcont blocks, combined with atomicMax into zeroed maxima)
 // in one launch, so that the two passes over HBM overlap.  Blocks below
-// b_blocks scan B; the others split row blockIdx.x - b_blocks of A.
+// b_blocks scan B; the others split rows 2 (blockIdx.x - b_blocks) and the
+// next of A.
 template <int NP, class FMT>
 __global__ __launch_bounds__(256) void split_prep_rowscaled_a_amax_b(
     const float *__restrict__ A, long lda, int rows_a, int K, bool vec_a, typename FMT::E *__restrict__ out_a,
@@ -522,8 +534,8 @@ __global__ __launch_bounds__(256) void split_prep_rowscaled_a_amax_b(
     if ((int)blockIdx.x < b_blocks)
         row_amax_rcont_block(B, ldb, rows_b, K, vec_b, kblocks_b, amax_b, blockIdx.x);
     else
-        prep_kcont_rowscaled_row<NP, FMT>(A, lda, rows_a, K, vec_a, out_a, plane_a, rows_pad_a, k_pad, amax_a,
-                                          (int)blockIdx.x - b_blocks);
+        prep_kcont_rowscaled_pair<NP, FMT>(A, lda, rows_a, K, vec_a, out_a, plane_a, rows_pad_a, k_pad, amax_a,
+                                           ((int)blockIdx.x - b_blocks) * 2);
 }
 
 __global__ __launch_bounds__(256) void grouped_row_amax(const PrepJob *__restrict__ jobs,
@@ -1297,9 +1309,9 @@ int prep_operand(const float *X, long ld, bool kcont, int rows, int K, typename 
     const bool vec = ugemm_aligned16(X) && (ld % 4 == 0);
     if constexpr (FMT::SCALED) {
         if (amax_done) goto planes;   // the row maxima were computed by an earlier launch
-        if (split && kcont && rows > 0 && K > 0 && k_pad <= 8192) {
-            // Row maxima and planes in one pass (one block per padded row).
-            hipLaunchKernelGGL((split_prep_kcont_rowscaled<NP, FMT>), dim3((unsigned)rows_pad), dim3(256), 0, s, X,
+        if (split && kcont && rows > 0 && K > 0 && k_pad <= 8192 && rows_pad % 2 == 0) {
+            // Row maxima and planes in one pass (one block per two padded rows).
+            hipLaunchKernelGGL((split_prep_kcont_rowscaled<NP, FMT>), dim3((unsigned)(rows_pad / 2)), dim3(256), 0, s, X,
                                ld, rows, K, vec, out, plane_stride, rows_pad, k_pad, amax);
             UGEMM_HIP_TRY(hipGetLastError());
             return UGEMM_OK;
@@ -1441,7 +1453,7 @@ int run_cfg(const Gemm<float> &g, hipStream_t s) {
         if (!fa && !fb && !g.ta && !g.tb && g.M > 0 && g.N > 0 && g.K > 0 && Kp <= 8192) {
             const long kblocks = (g.K + kAmaxRcontSpan - 1) / kAmaxRcontSpan, cblocks = (g.N + 255) / 256;
             const bool vec_a = ugemm_aligned16(g.A) && g.lda % 4 == 0, vec_b = ugemm_aligned16(g.B) && g.ldb % 4 == 0;
-            hipLaunchKernelGGL((split_prep_rowscaled_a_amax_b<NP, FMT>), dim3((unsigned)(cblocks * kblocks + Mp)),
+            hipLaunchKernelGGL((split_prep_rowscaled_a_amax_b<NP, FMT>), dim3((unsigned)(cblocks * kblocks + Mp / 2)),
                                dim3(256), 0, s, g.A, g.lda, g.M, g.K, vec_a, Ap, a_plane, (int)Mp, (int)Kp, amax_a,
                                g.B, g.ldb, g.N, vec_b, kblocks, amax_b, (int)(cblocks * kblocks));
             const hipError_t e = hipGetLastError();
