@@ -381,6 +381,14 @@ __global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restr
     const PrepJob j = jobs[block_job[blockIdx.x]];
     const int lb = (int)blockIdx.x - j.block0;
     typename FMT::E *out = static_cast<typename FMT::E *>(j.out);
+    if constexpr (FMT::SCALED) {
+        if (j.kcont == kPrepRowScaled) {
+            // The maxima buffer is the plan's own device memory, written here.
+            prep_kcont_rowscaled_pair<NP, FMT>(j.X, j.ld, j.rows, j.K, j.vec != 0, out, j.plane_stride, j.rows_pad,
+                                               j.k_pad, const_cast<unsigned *>(j.amax), lb * 2);
+            return;
+        }
+    }
     const int r0 = (lb % j.nbx) * 64, kb = (lb / j.nbx) * kPrepSpan;
     if (j.kcont) {
         for (int s = 0; s < kPrepSpan && (kb + s) * 128 < j.k_pad; ++s)

@@ -466,7 +466,7 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
     std::vector<long> a_off, b_off;
     long total = 0, blocks = 0;   // plane elements, preparation blocks
     auto add_job = [&](const float *X, long ld, bool kcont, int rows, int K, int rows_pad, int k_pad, long off,
-                       long plane, long amax_at) {
+                       long plane, long amax_at, bool row_scaled = false) {
         PrepJob j{};
         j.amax = scaled ? (const unsigned *)(intptr_t)amax_at : nullptr;
         j.X = X;
@@ -477,12 +477,12 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         j.K = K;
         j.rows_pad = rows_pad;
         j.k_pad = k_pad;
-        j.kcont = kcont ? 1 : 0;
+        j.kcont = row_scaled ? kPrepRowScaled : (kcont ? 1 : 0);
         j.vec = (K > 0 && ugemm_aligned16(X) && ld % 4 == 0) ? 1 : 0;
         j.block0 = (int)blocks;
         j.nbx = rows_pad / 64;
         const long ktiles = kcont ? (k_pad + 127) / 128 : k_pad / 64;
-        blocks += (long)(rows_pad / 64) * ((ktiles + kPrepSpan - 1) / kPrepSpan);
+        blocks += row_scaled ? rows_pad / 2 : (long)(rows_pad / 64) * ((ktiles + kPrepSpan - 1) / kPrepSpan);
         jobs.push_back(j);
     };
     for (int i = 0; i < (int)p->probs.size(); ++i) {
@@ -496,12 +496,15 @@ int build_split_schedule(ugemm_grouped_plan *p, int v) {
         total = bo + (split_b ? 0 : np * b_plane);
         const bool ta = (q.layout & 1) != 0, tb = (q.layout & 2) != 0;
         const long am_a = acount, am_b = acount + Mp;
+        // FP16X3 with a k-contiguous A of at most 8192 columns: A's maxima come
+        // with its split (two rows per block), not from the maximum pass.
+        const bool a_rows = scaled && !ta && Kp <= 8192 && Mp % 2 == 0;
         if (scaled) {
             acount += Mp + Np;
-            add_amax_job(q.A, q.lda, !ta, q.M, q.K, am_a);
+            if (!a_rows) add_amax_job(q.A, q.lda, !ta, q.M, q.K, am_a);
             add_amax_job(q.B, q.ldb, tb, q.N, q.K, am_b);
         }
-        add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane, am_a);
+        add_job(q.A, q.lda, !ta, q.M, q.K, (int)Mp, (int)Kp, ao, a_plane, am_a, a_rows);
         if (!split_b) add_job(q.B, q.ldb, tb, q.N, q.K, (int)Np, (int)Kp, bo, b_plane, am_b);
         GroupedSplitProblem g{};
         if (split_b) {

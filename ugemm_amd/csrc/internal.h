@@ -62,8 +62,12 @@ struct GroupedSplitTile {
 // its planes, and the first block of the job in the preparation grid.
 // A block takes kPrepSpan consecutive k tiles of one 64-row stripe: 64 x 128
 // tiles of a k-contiguous job, 64 x 64 tiles of a row-contiguous one (nbx
-// blocks along the rows either way).
+// blocks along the rows either way).  kcont == kPrepRowScaled (FP16X3,
+// k-contiguous, k_pad <= 8192): a block takes two whole rows, computes their
+// maxima (written to amax) and splits them, so the operand needs no
+// row-maximum pass.
 constexpr int kPrepSpan = 4;
+constexpr int kPrepRowScaled = 2;
 struct PrepJob {
     const float *X;
     long ld;
