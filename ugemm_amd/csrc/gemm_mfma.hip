@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void grouped_split_prep(const PrepJob *__restr
 // tiles of a k-contiguous operand, 64 x 64 transposing tiles of a
 // row-contiguous one; nbx blocks along the rows).  One launch instead of
 // two saves a launch and lets b's tiles start while a's last ones drain
-// (measured at 2048^3: 5-10% of a BF16x3 call).
+// (2048^3 BF16x3: 263 against 256 TF/s per call).
 template <int NP, class FMT>
 __global__ __launch_bounds__(256) void split_prep_pair(PrepJob a, PrepJob b) {
     static_assert(!FMT::SCALED, "row-scaled formats compute their maxima first");
