@@ -195,6 +195,25 @@ def test_fp16x3_matches_emulation(dist, ta, tb):
     assert ok, f"worst/bound {worst:.3g}"
 
 
+@pytest.mark.parametrize("K", [8192, 8190, 8200])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_fp16x3_row_scaled_split_at_k_limit(K, tb):
+    """The split of a k-contiguous A that computes its row maxima itself
+    (two rows per block, up to 8 passes of 32 K tiles: K padded to at most
+    8192; NN merges it with B's column maxima, NT runs it alone) against the
+    exact emulation, at and around its K limit (8200 takes the separate
+    row-maximum pass); 66 rows leave padding rows in the last pair."""
+    M, N = 66, 40
+    case = Case("R", "N", tb, M, N, K, 1.0, 0.0, seed=K * 7 + (tb == "T"), dist="wide")
+    C = run_host(case, ug.MODE_FP16X3)
+    Aop = orc.op_matrix("R", "N", case.A, M, K)
+    Bop = orc.op_matrix("R", tb, case.B, K, N)
+    emu = orc.fp16x3_product(Aop, Bop)
+    scale = np.abs(Aop.astype(np.float64)) @ np.abs(Bop.astype(np.float64))
+    ok, worst = orc.check_bound(C[:, :N], emu, scale, orc.tolerance(K))
+    assert ok, f"K {K} {tb}: worst/bound {worst:.3g}"
+
+
 def test_fp16x3_row_scales_extreme_rows():
     """Rows and columns of very different magnitude (2^-60 .. 2^60) and an
     all-zero row: each row's scale brings it into fp16 range independently."""
