@@ -426,3 +426,32 @@ def test_grouped_more_tiles_than_workgroups_two_plans():
     pa.destroy()
     pb.destroy()
     s.destroy()
+
+
+@pytest.mark.parametrize("variant", [1, 8, 10])
+def test_grouped_fp16x3_row_scaled_preparation_at_k_limit(variant):
+    """FP16X3 plans whose k-contiguous A (K padded to at most 8192) gets its
+    row maxima in its own preparation (two rows per block), beside problems
+    that take the row-maximum pass (K = 8200, and A transposed), against the
+    exact emulation of each product; plane-reading and split-B variants."""
+    mode = ug.MODE_FP16X3
+    specs = [("N", 66, 40, 8192), ("N", 130, 50, 8200), ("T", 70, 33, 300), ("N", 3, 7, 8190)]
+    cases = [DevCase(Case("R", ta, "N", M, N, K, 1.0, 0.0, seed=1000 + i, dist="wide"))
+             for i, (ta, M, N, K) in enumerate(specs)]
+    ug.force_variant(mode, variant)
+    try:
+        plan = ug.GroupedPlan([d.problem() for d in cases], mode=mode, capture=True)
+    finally:
+        ug.force_variant(mode, -1)
+    plan.run(None)
+    hip.synchronize()
+    for d in cases:
+        c = d.case
+        C = d.dC.download()
+        Aop = orc.op_matrix("R", c.ta, c.A, c.M, c.K)
+        Bop = orc.op_matrix("R", c.tb, c.B, c.K, c.N)
+        emu = orc.fp16x3_product(Aop, Bop)
+        scale = np.abs(Aop.astype(np.float64)) @ np.abs(Bop.astype(np.float64))
+        ok, worst = orc.check_bound(C[:, :c.N], emu, scale, orc.tolerance(c.K))
+        assert ok, f"variant {variant} {c.ta}N {c.M}x{c.N}x{c.K}: worst/bound {worst:.3g}"
+    plan.destroy()
